@@ -1,0 +1,224 @@
+// acx_api.hip -- the C ABI (include/acx.h): argument checks, kernel choice
+// and the SSP stage sequences.
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include "acx_device.h"
+#include "acx_internal.h"
+
+namespace acx {
+
+static thread_local char g_err[512] = "";
+
+int set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_check(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return ACX_OK;
+    return set_error(ACX_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+static int env_int(const char *name, int dflt)
+{
+    const char *s = getenv(name);
+    return (s && *s) ? atoi(s) : dflt;
+}
+
+static int check_grid(const acx_grid *g)
+{
+    if (!g) return set_error(ACX_EINVAL, "grid is NULL");
+    for (int d = 0; d < 3; ++d) {
+        if (g->n[d] < 4) return set_error(ACX_EINVAL, "n[%d] = %d: at least 4 cells per axis are required", d, g->n[d]);
+        if (g->bc[d] != ACX_PERIODIC && g->bc[d] != ACX_ADIABATIC)
+            return set_error(ACX_EINVAL, "bc[%d] = %d is not a boundary kind", d, g->bc[d]);
+    }
+    if ((size_t)g->n[0] * g->n[1] > 0x7fffffffu) return set_error(ACX_EINVAL, "plane too large");
+    if (g->system != ACX_WAVE2 && g->system != ACX_ACOUSTIC4)
+        return set_error(ACX_EINVAL, "system %d unknown", g->system);
+    if (!(g->h > 0)) return set_error(ACX_EINVAL, "h must be positive");
+    if (g->system == ACX_ACOUSTIC4) {
+        for (int d = 0; d < 3; ++d)
+            if (g->bc[d] != ACX_PERIODIC)
+                return set_error(ACX_EUNSUPPORTED, "ACOUSTIC4 supports periodic boundaries only");
+        if (!(g->rho0 > 0) || !(g->c0 > 0)) return set_error(ACX_EINVAL, "rho0 and c0 must be positive");
+    }
+    if (!(g->gamma_hi > 0 && g->gamma_hi <= 1) || !(g->gamma_lo >= 0 && g->gamma_lo <= 1))
+        return set_error(ACX_EINVAL, "linear weights out of range");
+    return ACX_OK;
+}
+
+static inline bool is_linear(const acx_grid *g)
+{
+    return g->gamma_hi == 1.0 && g->gamma_lo == 0.0 && !env_int("ACX_FORCE_CWENO", 0);
+}
+
+static inline int nvar_of(const acx_grid *g) { return g->system == ACX_ACOUSTIC4 ? 4 : 2; }
+
+static void fill_lin(LinArgs &a, const acx_grid *g)
+{
+    memset(&a, 0, sizeof(a));
+    a.nx = g->n[0]; a.ny = g->n[1]; a.nz = g->n[2];
+    int zc = env_int("ACX_ZC_LINEAR", 32);
+    a.zc = zc < 1 ? 1 : (zc > g->n[2] ? g->n[2] : zc);
+    for (int d = 0; d < 3; ++d) a.bc[d] = g->bc[d];
+    a.K = g->rho0 * g->c0 * g->c0;
+    a.invrho = 1.0 / g->rho0;
+    a.s = g->c0;
+    a.inv48h = 1.0 / (48.0 * g->h);
+    a.lapc = g->c0 * g->c0 / (48.0 * g->h * g->h);
+}
+
+static void fill_cw(CwArgs &a, const acx_grid *g)
+{
+    memset(&a, 0, sizeof(a));
+    a.nx = g->n[0]; a.ny = g->n[1]; a.nz = g->n[2];
+    int zc = env_int("ACX_ZC_CWENO", 32);
+    a.zc = zc < 1 ? 1 : (zc > g->n[2] ? g->n[2] : zc);
+    for (int d = 0; d < 3; ++d) a.bc[d] = g->bc[d];
+    a.K = g->rho0 * g->c0 * g->c0;
+    a.invrho = 1.0 / g->rho0;
+    a.s = g->c0;
+    a.inv_h = 1.0 / g->h;
+    a.c2 = g->c0 * g->c0 / (g->h * g->h);
+    // linear weights, reconstruction.c:55-59
+    a.g4 = g->gamma_hi;
+    a.g3c = (1.0 - g->gamma_hi) * g->gamma_lo;
+    a.g3s = (1. / 6.) * (1 - g->gamma_hi) * (1 - g->gamma_lo);
+    a.inv_g4 = 1.0 / g->gamma_hi;
+}
+
+enum { EPI_RHS = 0, EPI_AXPY = 1, EPI_COMB = 2, EPI_F32C = 3 };
+
+// One FP64 stage: OUT = a*U0 + b*Q + c*F(Q), epilogue chosen from (a, b).
+static int stage64(const acx_grid *g, const double *Q, const double *U0, double *OUT,
+                   double a, double b, double c, int epi, hipStream_t s)
+{
+    if (is_linear(g)) {
+        LinArgs la;
+        fill_lin(la, g);
+        la.q = Q; la.u0 = U0; la.out = OUT;
+        la.a = a; la.b = b; la.c = c;
+        return launch_linear(la, g->system, epi, false, s);
+    }
+    CwArgs ca;
+    fill_cw(ca, g);
+    ca.q = Q; ca.u0 = U0; ca.out = OUT;
+    ca.a = a; ca.b = b; ca.c = c;
+    return launch_cweno(ca, g->system, epi, false, false, s);
+}
+
+static int stage32c(const acx_grid *g, const float *HI, const float *D, const float *LO, float *OUT,
+                    float *OUT2, double b, double c, int final_stage, const uint8_t *flags, hipStream_t s)
+{
+    if (is_linear(g) && !flags) {
+        LinArgs la;
+        fill_lin(la, g);
+        la.q = HI; la.d = D; la.lo = LO; la.out = OUT; la.out2 = OUT2;
+        la.b = b; la.c = c; la.final_stage = final_stage;
+        return launch_linear(la, g->system, EPI_F32C, true, s);
+    }
+    CwArgs ca;
+    fill_cw(ca, g);
+    ca.q = HI; ca.d = D; ca.lo = LO; ca.out = OUT; ca.out2 = OUT2;
+    ca.b = b; ca.c = c; ca.final_stage = final_stage;
+    ca.flags = flags;
+    return launch_cweno(ca, g->system, EPI_F32C, true, flags != nullptr, s);
+}
+
+} // namespace acx
+
+using namespace acx;
+
+extern "C" {
+
+const char *acx_last_error(void) { return g_err; }
+int acx_version(void) { return 1; }
+
+double acx_cfl_dt(const acx_grid *g, double cfl)
+{
+    // dt = CFL h / (3 s_max), s_max = max(c0, c0) = c0 (LinearisedEuler/pde.c:66,87)
+    return (cfl * g->h) / (3.0 * g->c0);
+}
+
+int acx_rhs_f64(const acx_grid *g, const double *U, double *RHS, void *stream)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!U || !RHS || U == RHS) return set_error(ACX_EINVAL, "U and RHS must be distinct device arrays");
+    return stage64(g, U, nullptr, RHS, 0.0, 0.0, 1.0, EPI_RHS, (hipStream_t)stream);
+}
+
+int acx_stage_f64(const acx_grid *g, const double *Q, const double *U0, double *OUT,
+                  double a, double b, double c, void *stream)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!Q || !OUT || OUT == Q || (U0 && OUT == U0)) return set_error(ACX_EINVAL, "OUT must not alias Q or U0");
+    int epi;
+    if (a == 0.0 && b == 0.0 && c == 1.0) epi = EPI_RHS;
+    else if (a == 0.0 && b == 1.0) epi = EPI_AXPY;
+    else {
+        if (a != 0.0 && !U0) return set_error(ACX_EINVAL, "U0 is NULL while a != 0");
+        epi = EPI_COMB;
+        if (!U0) U0 = Q;   // a == 0: the base is unused
+    }
+    return stage64(g, Q, U0, OUT, a, b, c, epi, (hipStream_t)stream);
+}
+
+int acx_ssp_rks3_step_f64(const acx_grid *g, double *U, double *W0, double *W1, double dt, void *stream)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!U || !W0 || !W1 || U == W0 || U == W1 || W0 == W1) return set_error(ACX_EINVAL, "U, W0, W1 must be distinct");
+    hipStream_t s = (hipStream_t)stream;
+    // TSSSPRKS3, s = 4 (n = 2, r = 2): stage times 0, 1/2, 1, 1/2.
+    if ((rc = stage64(g, U, nullptr, W0, 0.0, 1.0, dt / 2, EPI_AXPY, s))) return rc;
+    if ((rc = stage64(g, W0, nullptr, W1, 0.0, 1.0, dt / 2, EPI_AXPY, s))) return rc;
+    if ((rc = stage64(g, W1, U, W0, 1. * 2 / 3, (2 - 1.) * 1. / 3, (2 - 1.) * dt / (2 * 3), EPI_COMB, s))) return rc;
+    return stage64(g, W0, nullptr, U, 0.0, 1.0, dt / 2, EPI_AXPY, s);
+}
+
+int acx_ssp_rks3_step_f32c(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
+                           float *LO_OUT, float *D0, float *D1, double dt, void *stream)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!HI || !LO || !HI_OUT || !LO_OUT || !D0 || !D1 || HI == HI_OUT || LO == LO_OUT)
+        return set_error(ACX_EINVAL, "f32c step: outputs must not alias the inputs");
+    hipStream_t s = (hipStream_t)stream;
+    // increments relative to the base state U = HI + LO:
+    //   D0 = dt/2 F(U); D1 = D0 + dt/2 F(U+D0); D0 = D1/3 + dt/6 F(U+D1);
+    //   U += D0 + dt/2 F(U+D0)   (two-sum into HI_OUT, LO_OUT)
+    if ((rc = stage32c(g, HI, nullptr, LO, D0, nullptr, 0.0, dt / 2, 0, nullptr, s))) return rc;
+    if ((rc = stage32c(g, HI, D0, LO, D1, nullptr, 1.0, dt / 2, 0, nullptr, s))) return rc;
+    if ((rc = stage32c(g, HI, D1, LO, D0, nullptr, 1.0 / 3.0, dt / 6, 0, nullptr, s))) return rc;
+    return stage32c(g, HI, D0, LO, HI_OUT, LO_OUT, 1.0, dt / 2, 1, nullptr, s);
+}
+
+} // extern "C"
+
+// selective (learned-indicator) step, declared in acx_learn.h
+extern "C" int acx_ssp_rks3_step_f32c_sel(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
+                                          float *LO_OUT, float *D0, float *D1, const uint8_t *flags,
+                                          double dt, void *stream)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!flags) return set_error(ACX_EINVAL, "flags is NULL");
+    if (!HI || !LO || !HI_OUT || !LO_OUT || !D0 || !D1 || HI == HI_OUT || LO == LO_OUT)
+        return set_error(ACX_EINVAL, "f32c step: outputs must not alias the inputs");
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = stage32c(g, HI, nullptr, LO, D0, nullptr, 0.0, dt / 2, 0, flags, s))) return rc;
+    if ((rc = stage32c(g, HI, D0, LO, D1, nullptr, 1.0, dt / 2, 0, flags, s))) return rc;
+    if ((rc = stage32c(g, HI, D1, LO, D0, nullptr, 1.0 / 3.0, dt / 6, 0, flags, s))) return rc;
+    return stage32c(g, HI, D0, LO, HI_OUT, LO_OUT, 1.0, dt / 2, 1, flags, s);
+}

@@ -1,0 +1,120 @@
+// acx_device.h -- device-side building blocks shared by the gfx950 kernels.
+//
+// Ghost cells are never materialised: a neighbour index outside [0, n) is
+// mapped back into the owned range the way the reference's DMDA ghost fill
+// does it (3d/weno4/plane_wave/hype.c:87-104 periodic wrap;
+// 3d/weno4/plane_wave/rhs_function.c:48-135 odd reflection of component 0
+// for adiabatic walls, applied axis by axis, which makes a corner ghost
+// (-1)^m times the reflected owned value for m reflected axes).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace acx {
+
+// Maps a (possibly ghost) index along one axis.  `sign` is multiplied by -1
+// for a reflection.  Indices further than 3 cells outside a non-periodic axis
+// are only reached by threads whose results are discarded; they are clamped
+// so that every load stays inside the array.
+__device__ __forceinline__ int map_axis(int i, int n, int periodic, float &sign)
+{
+    if (i >= 0 && i < n) return i;
+    if (periodic) {
+        while (i < 0) i += n;
+        while (i >= n) i -= n;
+        return i;
+    }
+    if (i < 0) {
+        sign = -sign;
+        int r = -1 - i;
+        return r < n ? r : n - 1;
+    }
+    sign = -sign;
+    int r = 2 * n - 1 - i;
+    return r >= 0 ? r : 0;
+}
+
+template <typename T> struct Vec2 { T a, b; };
+
+// AoS cell access (4 components: p,u,v,w; or 2 components: u,v)
+template <typename T, int NV> struct Cell { T v[NV]; };
+
+template <typename T, int NV>
+__device__ __forceinline__ Cell<T, NV> load_cell(const T *__restrict__ base, size_t idx)
+{
+    Cell<T, NV> c;
+    if constexpr (sizeof(T) == 8 && NV == 4) {
+        const double2 *p = reinterpret_cast<const double2 *>(base + idx * 4);
+        double2 a = p[0], b = p[1];
+        c.v[0] = a.x; c.v[1] = a.y; c.v[2] = b.x; c.v[3] = b.y;
+    } else if constexpr (sizeof(T) == 8 && NV == 2) {
+        const double2 *p = reinterpret_cast<const double2 *>(base + idx * 2);
+        double2 a = p[0];
+        c.v[0] = a.x; c.v[1] = a.y;
+    } else if constexpr (sizeof(T) == 4 && NV == 4) {
+        const float4 *p = reinterpret_cast<const float4 *>(base + idx * 4);
+        float4 a = p[0];
+        c.v[0] = a.x; c.v[1] = a.y; c.v[2] = a.z; c.v[3] = a.w;
+    } else {
+        const float2 *p = reinterpret_cast<const float2 *>(base + idx * 2);
+        float2 a = p[0];
+        c.v[0] = a.x; c.v[1] = a.y;
+    }
+    return c;
+}
+
+template <typename T, int NV>
+__device__ __forceinline__ void store_cell(T *__restrict__ base, size_t idx, const Cell<T, NV> &c)
+{
+    if constexpr (sizeof(T) == 8 && NV == 4) {
+        double2 *p = reinterpret_cast<double2 *>(base + idx * 4);
+        p[0] = make_double2(c.v[0], c.v[1]);
+        p[1] = make_double2(c.v[2], c.v[3]);
+    } else if constexpr (sizeof(T) == 8 && NV == 2) {
+        double2 *p = reinterpret_cast<double2 *>(base + idx * 2);
+        p[0] = make_double2(c.v[0], c.v[1]);
+    } else if constexpr (sizeof(T) == 4 && NV == 4) {
+        float4 *p = reinterpret_cast<float4 *>(base + idx * 4);
+        p[0] = make_float4(c.v[0], c.v[1], c.v[2], c.v[3]);
+    } else {
+        float2 *p = reinterpret_cast<float2 *>(base + idx * 2);
+        p[0] = make_float2(c.v[0], c.v[1]);
+    }
+}
+
+// Linear-scheme face stencils folded into 7-point cell stencils.
+// With the shipped weights (gamma_hi = 1, gamma_lo = 0) the CWENO4 face
+// averages are q_L(i+1/2) = (q_{i-2} - 6q_{i-1} + 20q_i + 10q_{i+1} - q_{i+2})/24
+// and the mirror image q_R; the flux-difference of the LLF / central fluxes
+// then needs only
+//   A7 = 24 * [ (q_L+q_R)(i+1/2) - (q_L+q_R)(i-1/2) ]
+//      = 37(q1-q-1) - 8(q2-q-2) + (q3-q-3)
+//   B7 = 24 * [ (q_R-q_L)(i+1/2) - (q_R-q_L)(i-1/2) ]
+//      = (q3+q-3) - 6(q2+q-2) + 15(q1+q-1) - 20 q0
+// and for the central flux of face-averaged normal gradients (WAVE2)
+//   L7 = 48 h^2 * div = -(q3+q-3) + 2(q2+q-2) + 49(q1+q-1) - 100 q0.
+template <typename T>
+__device__ __forceinline__ T stencil_A7(T m3, T m2, T m1, T p1, T p2, T p3)
+{
+    return T(37) * (p1 - m1) - T(8) * (p2 - m2) + (p3 - m3);
+}
+template <typename T>
+__device__ __forceinline__ T stencil_B7(T m3, T m2, T m1, T q0, T p1, T p2, T p3)
+{
+    return (p3 + m3) - T(6) * (p2 + m2) + T(15) * (p1 + m1) - T(20) * q0;
+}
+template <typename T>
+__device__ __forceinline__ T stencil_L7(T m3, T m2, T m1, T q0, T p1, T p2, T p3)
+{
+    return T(2) * (p2 + m2) - (p3 + m3) + T(49) * (p1 + m1) - T(100) * q0;
+}
+
+// Error-free sum of two floats (Knuth's TwoSum): s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
+{
+    s = a + b;
+    float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+} // namespace acx

@@ -1,0 +1,49 @@
+// acx_internal.h -- host-side declarations shared by the library's
+// translation units (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include "../../include/acx.h"
+
+namespace acx {
+
+// Arguments of the fused linear stage kernel (acx_linear.hip).
+struct LinArgs {
+    const void *q;     // stage input (F32C: the HI part of the base state)
+    const void *d;     // F32C: increment added to q before reconstruction (may be null)
+    const void *u0;    // COMB: base state
+    const void *lo;    // F32C final stage: LO part of the base state
+    void *out;         // result
+    void *out2;        // F32C final stage: new LO
+    int nx, ny, nz, zc;
+    int bc[3];
+    int final_stage;
+    double a, b, c;    // epilogue coefficients
+    double K, invrho, s, inv48h, lapc;
+};
+
+// Arguments of the fused nonlinear CWENO4 stage kernel (acx_cweno.hip).
+struct CwArgs {
+    const void *q;
+    const void *d;
+    const void *u0;
+    const void *lo;
+    void *out;
+    void *out2;
+    const uint8_t *flags;   // selective mode: per-cell bitmask, bit v set => full CWENO for component v
+    int nx, ny, nz, zc;
+    int bc[3];
+    int final_stage;
+    double a, b, c;
+    double K, invrho, s, inv_h, c2;
+    double g4, g3c, g3s, inv_g4;   // linear weights: gamma_R4, gamma_R3[0], gamma_R3[1..6], 1/gamma_R4
+};
+
+int launch_linear(const LinArgs &a, int sys, int epi, bool fp32, hipStream_t s);
+int launch_cweno(const CwArgs &a, int sys, int epi, bool fp32, bool selective, hipStream_t s);
+
+int set_error(int code, const char *fmt, ...);
+int hip_check(hipError_t e, const char *what);
+
+} // namespace acx

@@ -1,0 +1,328 @@
+// acx_misc.hip -- FP64 reductions, the CTCS leapfrog sibling and host-side
+// synthetic fields.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+#include "acx_device.h"
+#include "acx_internal.h"
+
+namespace acx {
+
+// ---- deterministic two-pass FP64 reductions ------------------------------
+// pass 1: each block reduces a grid-stride slice to (sum, max) with wavefront
+// shuffles; pass 2: one block reduces the partials in a fixed order.
+constexpr int RED_THREADS = 256;
+constexpr int RED_BLOCKS = 1024;
+
+__device__ __forceinline__ void wave_reduce(double &s, double &m)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_down(s, off, 64);
+        m = fmax(m, __shfl_down(m, off, 64));
+    }
+}
+
+__device__ __forceinline__ void block_reduce(double &s, double &m)
+{
+    __shared__ double ss[RED_THREADS / 64], sm[RED_THREADS / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    wave_reduce(s, m);
+    if (lane == 0) { ss[wv] = s; sm[wv] = m; }
+    __syncthreads();
+    if (wv == 0) {
+        s = lane < RED_THREADS / 64 ? ss[lane] : 0.0;
+        m = lane < RED_THREADS / 64 ? sm[lane] : 0.0;
+        wave_reduce(s, m);
+    }
+}
+
+// mode 0: x = X[i] (component filter); mode 1: x = (hi + lo) - B, also sums B^2
+__global__ void __launch_bounds__(RED_THREADS)
+k_reduce_pass1(const double *__restrict__ X, size_t n, int nvar, int comp, double *__restrict__ part)
+{
+    double s = 0.0, m = 0.0;
+    for (size_t i = blockIdx.x * (size_t)RED_THREADS + threadIdx.x; i < n; i += (size_t)gridDim.x * RED_THREADS) {
+        if (comp >= 0 && (int)(i % nvar) != comp) continue;
+        double v = X[i];
+        s += v * v;
+        m = fmax(m, fabs(v));
+    }
+    block_reduce(s, m);
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = s; part[2 * blockIdx.x + 1] = m; }
+}
+
+__global__ void __launch_bounds__(RED_THREADS)
+k_diff_pass1(const float *__restrict__ HI, const float *__restrict__ LO, const double *__restrict__ B, size_t n,
+             double *__restrict__ part)
+{
+    double s = 0.0, m = 0.0, sb = 0.0, mb = 0.0;
+    for (size_t i = blockIdx.x * (size_t)RED_THREADS + threadIdx.x; i < n; i += (size_t)gridDim.x * RED_THREADS) {
+        double a = (double)HI[i] + (LO ? (double)LO[i] : 0.0);
+        double d = a - B[i];
+        s += d * d;
+        m = fmax(m, fabs(d));
+        sb += B[i] * B[i];
+    }
+    block_reduce(s, m);
+    __syncthreads();
+    block_reduce(sb, mb);
+    if (threadIdx.x == 0) { part[3 * blockIdx.x] = s; part[3 * blockIdx.x + 1] = m; part[3 * blockIdx.x + 2] = sb; }
+}
+
+__global__ void __launch_bounds__(RED_THREADS)
+k_energy_pass1(const double *__restrict__ U, size_t ncell, double invK, double rho0, double *__restrict__ part)
+{
+    double s = 0.0, m = 0.0;
+    for (size_t i = blockIdx.x * (size_t)RED_THREADS + threadIdx.x; i < ncell; i += (size_t)gridDim.x * RED_THREADS) {
+        const double2 *p = reinterpret_cast<const double2 *>(U + 4 * i);
+        double2 a = p[0], b = p[1];
+        double e = a.x * a.x * invK + rho0 * (a.y * a.y + b.x * b.x + b.y * b.y);
+        s += e;
+        m = fmax(m, e);
+    }
+    block_reduce(s, m);
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = s; part[2 * blockIdx.x + 1] = m; }
+}
+
+__global__ void __launch_bounds__(RED_THREADS)
+k_reduce_pass2(const double *__restrict__ part, int nblk, int stride, double *__restrict__ out)
+{
+    // fixed-order accumulation of the block partials (columns 0: sum, 1: max, 2: sum)
+    double s = 0.0, m = 0.0, s2 = 0.0, dummy = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += RED_THREADS) {
+        s += part[stride * i];
+        m = fmax(m, part[stride * i + 1]);
+        if (stride > 2) s2 += part[stride * i + 2];
+    }
+    block_reduce(s, m);
+    __syncthreads();
+    block_reduce(s2, dummy);
+    if (threadIdx.x == 0) { out[0] = s; out[1] = m; out[2] = s2; }
+}
+
+// Partials + result buffer, allocated once per device on first use.
+static double *scratch_for_device()
+{
+    static double *buf[64] = {nullptr};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    if (!buf[dev]) {
+        double *p = nullptr;
+        if (hipMalloc(&p, (3 * RED_BLOCKS + 4) * sizeof(double)) != hipSuccess) return nullptr;
+        buf[dev] = p;
+    }
+    return buf[dev];
+}
+
+static int finish(double *part, int stride, double *host_out, int nout, hipStream_t s)
+{
+    double *res = part + 3 * RED_BLOCKS;
+    hipLaunchKernelGGL(k_reduce_pass2, dim3(1), dim3(RED_THREADS), 0, s, part, RED_BLOCKS, stride, res);
+    int rc = hip_check(hipGetLastError(), "reduce pass 2");
+    if (rc) return rc;
+    double tmp[3];
+    rc = hip_check(hipMemcpyAsync(tmp, res, 3 * sizeof(double), hipMemcpyDeviceToHost, s), "reduce copy");
+    if (rc) return rc;
+    rc = hip_check(hipStreamSynchronize(s), "reduce sync");
+    if (rc) return rc;
+    memcpy(host_out, tmp, nout * sizeof(double));
+    return ACX_OK;
+}
+
+// ---- CTCS leapfrog (3d/ctcs/point_source/wave.c:138-215) -----------------
+
+__device__ __forceinline__ double ctcs_source(double t, double f0)
+{
+    const double t0 = 4.0 / f0;   // wave.c:234-240
+    return -2.0 * (t - t0) * f0 * f0 * exp(-1.0 * f0 * f0 * (t - t0) * (t - t0));
+}
+
+template <int ST>
+__global__ void __launch_bounds__(256)
+k_ctcs(const double *__restrict__ Pold, const double *__restrict__ P, double *__restrict__ Pnew,
+       const double *__restrict__ C, acx_ctcs c, int it, double *__restrict__ rcv)
+{
+    const int nx = c.n[0], ny = c.n[1], nz = c.n[2];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= nx) return;
+    auto W = [](int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); };
+    auto at = [&](int a, int b, int e) { return P[((size_t)W(e, nz) * ny + W(b, ny)) * nx + W(a, nx)]; };
+    const double invdx2 = 1.0 / (c.dx * c.dx), invdy2 = 1.0 / (c.dy * c.dy), invdz2 = 1.0 / (c.dz * c.dz);
+    const double p0 = at(i, j, k);
+    double pxx, pyy, pzz;
+    if (ST == 1) {
+        pxx = (at(i + 1, j, k) - 2.0 * p0 + at(i - 1, j, k)) * invdx2;
+        pyy = (at(i, j + 1, k) - 2.0 * p0 + at(i, j - 1, k)) * invdy2;
+        pzz = (at(i, j, k + 1) - 2.0 * p0 + at(i, j, k - 1)) * invdz2;
+    } else {
+        pxx = (-1.0 / 12 * at(i + 2, j, k) + 4.0 / 3.0 * at(i + 1, j, k) - 5.0 / 2.0 * p0 + 4.0 / 3.0 * at(i - 1, j, k) -
+               1.0 / 12.0 * at(i - 2, j, k)) * invdx2;
+        pyy = (-1.0 / 12 * at(i, j + 2, k) + 4.0 / 3.0 * at(i, j + 1, k) - 5.0 / 2.0 * p0 + 4.0 / 3.0 * at(i, j - 1, k) -
+               1.0 / 12.0 * at(i, j - 2, k)) * invdy2;
+        pzz = (-1.0 / 12 * at(i, j, k + 2) + 4.0 / 3.0 * at(i, j, k + 1) - 5.0 / 2.0 * p0 + 4.0 / 3.0 * at(i, j, k - 1) -
+               1.0 / 12.0 * at(i, j, k - 2)) * invdz2;
+    }
+    const size_t o = ((size_t)k * ny + j) * nx + i;
+    const double cc = C[o], dt = c.dt;
+    double pn = 2.0 * p0 - Pold[o] + cc * cc * dt * dt * (pxx + pyy + pzz);
+    if (i == c.src[0] && j == c.src[1] && k == c.src[2]) pn += ctcs_source(it * dt, c.f0) / (c.dx * c.dy * c.dz) * dt * dt;
+    if (i == c.rcv[0] && j == c.rcv[1] && k == c.rcv[2]) rcv[0] = p0;
+    Pnew[o] = pn;
+}
+
+// ---- host fields ---------------------------------------------------------
+
+static inline uint64_t splitmix64(uint64_t &s)
+{
+    uint64_t z = (s += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+static inline double unif(uint64_t &s) { return (splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+// 1D cell averages of exp(-(x-c)^2/(2 sigma^2)) on the periodic unit interval
+static void gauss_1d(int n, double h, double c, double sigma, std::vector<double> &out)
+{
+    out.assign(n, 0.0);
+    const double k = sigma * sqrt(M_PI / 2.0) / h, r = 1.0 / (sigma * sqrt(2.0));
+    for (int i = 0; i < n; ++i) {
+        const double a = i * h, b = (i + 1) * h;
+        double acc = 0.0;
+        for (int img = -1; img <= 1; ++img) acc += erf((b - c + img) * r) - erf((a - c + img) * r);
+        out[i] = k * acc;
+    }
+}
+
+} // namespace acx
+
+using namespace acx;
+
+extern "C" {
+
+int acx_norms_f64(const double *X, size_t ncells, int nvar, int comp, double *out, void *stream)
+{
+    if (!X || !out || nvar < 1 || comp >= nvar) return set_error(ACX_EINVAL, "norms: bad arguments");
+    double *part = scratch_for_device();
+    if (!part) return set_error(ACX_EHIP, "norms: scratch allocation failed");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_reduce_pass1, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, X, ncells * nvar, nvar, comp, part);
+    int rc = hip_check(hipGetLastError(), "norms pass 1");
+    if (rc) return rc;
+    double r[3];
+    if ((rc = finish(part, 2, r, 3, s))) return rc;
+    out[0] = r[0];
+    out[1] = r[1];
+    return ACX_OK;
+}
+
+int acx_diff_f32c_f64(const float *HI, const float *LO, const double *B, size_t n, double *out, void *stream)
+{
+    if (!HI || !B || !out) return set_error(ACX_EINVAL, "diff: bad arguments");
+    double *part = scratch_for_device();
+    if (!part) return set_error(ACX_EHIP, "diff: scratch allocation failed");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_diff_pass1, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, HI, LO, B, n, part);
+    int rc = hip_check(hipGetLastError(), "diff pass 1");
+    if (rc) return rc;
+    double r[3];
+    if ((rc = finish(part, 3, r, 3, s))) return rc;
+    out[0] = r[2] > 0 ? sqrt(r[0] / r[2]) : sqrt(r[0]);
+    out[1] = r[1];
+    return ACX_OK;
+}
+
+int acx_energy_f64(const acx_grid *g, const double *U, double *out, void *stream)
+{
+    if (!g || !U || !out || g->system != ACX_ACOUSTIC4) return set_error(ACX_EINVAL, "energy: ACOUSTIC4 state required");
+    double *part = scratch_for_device();
+    if (!part) return set_error(ACX_EHIP, "energy: scratch allocation failed");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nc = (size_t)g->n[0] * g->n[1] * g->n[2];
+    const double K = g->rho0 * g->c0 * g->c0;
+    hipLaunchKernelGGL(k_energy_pass1, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, U, nc, 1.0 / K, g->rho0, part);
+    int rc = hip_check(hipGetLastError(), "energy pass 1");
+    if (rc) return rc;
+    double r[3];
+    if ((rc = finish(part, 2, r, 3, s))) return rc;
+    out[0] = 0.5 * r[0] * g->h * g->h * g->h;
+    return ACX_OK;
+}
+
+int acx_ctcs_update_f64(const acx_ctcs *c, const double *POLD, const double *P, double *PNEW, const double *C,
+                        int it, double *rcv_out, void *stream)
+{
+    if (!c || !POLD || !P || !PNEW || !C || PNEW == P) return set_error(ACX_EINVAL, "ctcs: bad arguments");
+    for (int d = 0; d < 3; ++d)
+        if (c->n[d] < 5) return set_error(ACX_EINVAL, "ctcs: at least 5 cells per axis");
+    if (c->stencil != 1 && c->stencil != 2) return set_error(ACX_EINVAL, "stencil width can be either 1 or 2");
+    double *part = scratch_for_device();
+    if (!part) return set_error(ACX_EHIP, "ctcs: scratch allocation failed");
+    hipStream_t s = (hipStream_t)stream;
+    double *rcv = part + 3 * RED_BLOCKS + 3;
+    dim3 blk(256), grd((c->n[0] + 255) / 256, c->n[1], c->n[2]);
+    if (c->stencil == 1) hipLaunchKernelGGL(k_ctcs<1>, grd, blk, 0, s, POLD, P, PNEW, C, *c, it, rcv);
+    else hipLaunchKernelGGL(k_ctcs<2>, grd, blk, 0, s, POLD, P, PNEW, C, *c, it, rcv);
+    int rc = hip_check(hipGetLastError(), "ctcs launch");
+    if (rc) return rc;
+    if (rcv_out) {
+        if ((rc = hip_check(hipMemcpyAsync(rcv_out, rcv, sizeof(double), hipMemcpyDeviceToHost, s), "ctcs rcv copy"))) return rc;
+        if ((rc = hip_check(hipStreamSynchronize(s), "ctcs sync"))) return rc;
+    }
+    return ACX_OK;
+}
+
+int acx_field_gaussians(const acx_grid *g, uint64_t seed, int npulse, double amin, double amax, double smin,
+                        double smax, double *U)
+{
+    if (!g || !U || npulse < 0) return set_error(ACX_EINVAL, "field: bad arguments");
+    const int nx = g->n[0], ny = g->n[1], nz = g->n[2], nv = g->system == ACX_ACOUSTIC4 ? 4 : 2;
+    const size_t nc = (size_t)nx * ny * nz;
+    memset(U, 0, nc * nv * sizeof(double));
+    uint64_t st = seed;
+    std::vector<double> fx, fy, fz;
+    for (int q = 0; q < npulse; ++q) {
+        const double A = amin + (amax - amin) * unif(st);
+        const double sg = smin + (smax - smin) * unif(st);
+        const double cx = unif(st), cy = unif(st), cz = unif(st);
+        gauss_1d(nx, 1.0 / nx, cx, sg, fx);
+        gauss_1d(ny, 1.0 / ny, cy, sg, fy);
+        gauss_1d(nz, 1.0 / nz, cz, sg, fz);
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j) {
+                const double a = A * fz[k] * fy[j];
+                double *row = U + ((size_t)k * ny + j) * nx * nv;
+                for (int i = 0; i < nx; ++i) row[(size_t)i * nv] += a * fx[i];
+            }
+    }
+    return ACX_OK;
+}
+
+int acx_field_plane_wave(const acx_grid *g, const int k[3], double A, double t, double *U)
+{
+    if (!g || !U || g->system != ACX_ACOUSTIC4) return set_error(ACX_EINVAL, "plane wave: ACOUSTIC4 grid required");
+    const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+    const double kk = sqrt((double)k[0] * k[0] + (double)k[1] * k[1] + (double)k[2] * k[2]);
+    if (kk == 0) return set_error(ACX_EINVAL, "plane wave: k = 0");
+    const double w = 2.0 * M_PI * kk * g->c0;
+    // cell averages: point value at the centre times prod sinc(pi k_d h_d)
+    auto sinc = [](double x) { return x == 0 ? 1.0 : sin(x) / x; };
+    const double damp = sinc(M_PI * k[0] / nx) * sinc(M_PI * k[1] / ny) * sinc(M_PI * k[2] / nz);
+    const double ap = A * cos(w * t) * damp, au = -(A / (g->rho0 * g->c0)) * sin(w * t) * damp;
+    for (int c = 0; c < nz; ++c)
+        for (int b = 0; b < ny; ++b)
+            for (int a = 0; a < nx; ++a) {
+                const double th = 2.0 * M_PI * (k[0] * (a + 0.5) / nx + k[1] * (b + 0.5) / ny + k[2] * (c + 0.5) / nz);
+                double *o = U + (((size_t)c * ny + b) * nx + a) * 4;
+                const double sn = sin(th), cs = cos(th);
+                o[0] = ap * sn;
+                o[1] = au * (k[0] / kk) * cs;
+                o[2] = au * (k[1] / kk) * cs;
+                o[3] = au * (k[2] / kk) * cs;
+            }
+    return ACX_OK;
+}
+
+} // extern "C"

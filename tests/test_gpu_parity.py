@@ -1,0 +1,132 @@
+"""GPU parity of the fused RHS / SSP stage kernels against the FP64 oracle.
+
+Each case prints its name, shape and seed before the GPU call so that a
+failing run names it.  Tolerance: relative L2 <= 1e-12 (FP64 kernels).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def make_field(acx, grid, seed, kind):
+    rng = np.random.default_rng(seed)
+    shape = grid.shape()
+    if kind == "smooth":
+        U = acx.field_gaussians(grid, seed, npulse=3, amp=(0.5, 1.0), sigma=(0.1, 0.3))
+        for v in range(1, grid.nvar):
+            U[..., v] = np.roll(U[..., 0], 3 * v, axis=v % 3) * (0.3 * v)
+        return U
+    if kind == "rough":
+        return rng.standard_normal(shape)
+    if kind == "step":
+        nz, ny, nx, nv = shape
+        z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        inside = ((x - nx / 2) ** 2 + 1.3 * (y - ny / 2) ** 2 + 0.7 * (z - nz / 2) ** 2) < (min(nx, ny) / 3) ** 2
+        U = np.zeros(shape)
+        for v in range(nv):
+            U[..., v] = np.where(inside, 1.0 + 0.25 * v, 0.1) + 0.01 * rng.standard_normal(shape[:3])
+        return U
+    raise ValueError(kind)
+
+
+def orc_grid(orc, grid):
+    return orc.grid3d(tuple(grid.n), grid.h, grid.system, (grid.gamma_hi, grid.gamma_lo), grid.c0, grid.rho0,
+                      0.4, tuple(grid.bc))
+
+
+CASES = [
+    # name, n, system, gamma, bc, kind
+    ("ac4-lin-smooth", (70, 18, 12), 1, (1.0, 0.0), (0, 0, 0), "smooth"),
+    ("ac4-lin-rough", (24, 20, 16), 1, (1.0, 0.0), (0, 0, 0), "rough"),
+    ("ac4-nl-smooth", (24, 20, 16), 1, (0.85, 0.85), (0, 0, 0), "smooth"),
+    ("ac4-nl-rough", (40, 30, 12), 1, (0.85, 0.85), (0, 0, 0), "rough"),
+    ("ac4-nl-step", (32, 32, 10), 1, (0.85, 0.85), (0, 0, 0), "step"),
+    ("ac4-nl-g05", (20, 16, 14), 1, (0.5, 0.3), (0, 0, 0), "rough"),
+    ("w2-lin-per", (24, 20, 16), 0, (1.0, 0.0), (0, 0, 0), "rough"),
+    ("w2-lin-adia", (24, 20, 16), 0, (1.0, 0.0), (1, 1, 1), "rough"),
+    ("w2-lin-mixed", (70, 12, 10), 0, (1.0, 0.0), (0, 1, 1), "rough"),
+    ("w2-nl-per", (24, 20, 16), 0, (0.85, 0.85), (0, 0, 0), "rough"),
+    ("w2-nl-adia", (24, 20, 16), 0, (0.85, 0.85), (1, 1, 1), "rough"),
+]
+
+
+@pytest.mark.parametrize("name,n,system,gamma,bc,kind", CASES, ids=[c[0] for c in CASES])
+def test_rhs_parity(name, n, system, gamma, bc, kind, acx, orc, torch_gpu):
+    torch = torch_gpu
+    grid = acx.Grid(n=n, h=1.0 / n[0], system=system, bc=bc, gamma_hi=gamma[0], gamma_lo=gamma[1])
+    U = make_field(acx, grid, 1234, kind)
+    print(f"[case] rhs {name} n={n} seed=1234", flush=True)
+    ref = orc.rhs3d(orc_grid(orc, grid), U, faithful=True)
+    Ud = torch.from_numpy(U).cuda()
+    out = acx.rhs(grid, Ud)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    err = rel_l2(got, ref)
+    print(f"[case] rhs {name} rel_l2={err:.3e}", flush=True)
+    assert np.isfinite(got).all()
+    assert err <= TOL, f"{name}: rel L2 {err:.3e}"
+
+
+@pytest.mark.parametrize("name,n,system,gamma,bc,kind", [CASES[0], CASES[1], CASES[3], CASES[7]],
+                         ids=["ac4-lin-smooth", "ac4-lin-rough", "ac4-nl-rough", "w2-lin-adia"])
+def test_forced_general_kernel_matches(name, n, system, gamma, bc, kind, acx, orc, torch_gpu, monkeypatch):
+    """The general CWENO kernel run with the linear weights must agree with the oracle too."""
+    torch = torch_gpu
+    monkeypatch.setenv("ACX_FORCE_CWENO", "1")
+    grid = acx.Grid(n=n, h=1.0 / n[0], system=system, bc=bc, gamma_hi=gamma[0], gamma_lo=gamma[1])
+    U = make_field(acx, grid, 99, kind)
+    print(f"[case] forced-cweno {name} n={n}", flush=True)
+    ref = orc.rhs3d(orc_grid(orc, grid), U)
+    got = acx.rhs(grid, torch.from_numpy(U).cuda()).cpu().numpy()
+    err = rel_l2(got, ref)
+    print(f"[case] forced-cweno {name} rel_l2={err:.3e}", flush=True)
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("gamma", [(1.0, 0.0), (0.85, 0.85)])
+def test_rks3_step_parity(gamma, acx, orc, torch_gpu):
+    torch = torch_gpu
+    n = (24, 20, 18)
+    grid = acx.Grid(n=n, h=1.0 / 24, system=acx.ACOUSTIC4, gamma_hi=gamma[0], gamma_lo=gamma[1])
+    U = make_field(acx, grid, 7, "rough")
+    dt = 0.4 * grid.h / 3.0
+    print(f"[case] rks3 step gamma={gamma}", flush=True)
+    ref = U.copy()
+    g = orc_grid(orc, grid)
+    for _ in range(3):
+        orc.ts_solve("orc_rhs3d_cb_faithful", g, ref, dt, dt, 0, 4)
+    Ud = torch.from_numpy(U).cuda()
+    for _ in range(3):
+        acx.ssp_rks3_step(grid, Ud, dt)
+    got = Ud.cpu().numpy()
+    err = rel_l2(got, ref)
+    print(f"[case] rks3 gamma={gamma} rel_l2={err:.3e}", flush=True)
+    assert err <= TOL
+
+
+def test_stage_epilogues(acx, orc, torch_gpu):
+    """OUT = a*U0 + b*Q + c*F(Q) for the three epilogue forms."""
+    torch = torch_gpu
+    n = (20, 16, 12)
+    grid = acx.Grid(n=n, h=1.0 / 20, system=acx.ACOUSTIC4)
+    rng = np.random.default_rng(5)
+    Q = rng.standard_normal(grid.shape())
+    U0 = rng.standard_normal(grid.shape())
+    F = orc.rhs3d(orc_grid(orc, grid), Q)
+    Qd, U0d = torch.from_numpy(Q).cuda(), torch.from_numpy(U0).cuda()
+    for (a, b, c) in [(0.0, 0.0, 1.0), (0.0, 1.0, 0.25), (2.0 / 3.0, 1.0 / 3.0, 0.125)]:
+        out = torch.empty_like(Qd)
+        acx.stage(grid, Qd, U0d if a else None, out, a, b, c)
+        ref = a * U0 + b * Q + c * F
+        err = rel_l2(out.cpu().numpy(), ref)
+        print(f"[case] stage a={a} b={b} c={c} rel_l2={err:.3e}", flush=True)
+        assert err <= TOL
