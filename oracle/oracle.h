@@ -146,6 +146,9 @@ double orc_ctcs3d_update(const orc_ctcs3d *c, const double *Pold, const double *
 /* Runs nt steps from rest; trace[nt] gets the receiver samples. */
 void orc_ctcs3d_run(const orc_ctcs3d *c, int nt, double *trace);
 
+/* OpenMP thread count used by the oracle (0: library default). */
+void orc_set_threads(int n);
+
 /* ---- reductions ---------------------------------------------------------- */
 double orc_norm2(size_t n, const double *a);
 double orc_norminf(size_t n, const double *a);

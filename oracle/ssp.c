@@ -15,6 +15,18 @@
 #include <string.h>
 #include <stdlib.h>
 #include "oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
 
 static void axpy(size_t n, double *y, double a, const double *x)
 {
