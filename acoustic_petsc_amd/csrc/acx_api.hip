@@ -67,7 +67,7 @@ static void fill_lin(LinArgs &a, const acx_grid *g)
 {
     memset(&a, 0, sizeof(a));
     a.nx = g->n[0]; a.ny = g->n[1]; a.nz = g->n[2];
-    int zc = env_int("ACX_ZC_LINEAR", 32);
+    int zc = env_int("ACX_ZC_LINEAR", 64);
     a.zc = zc < 1 ? 1 : (zc > g->n[2] ? g->n[2] : zc);
     for (int d = 0; d < 3; ++d) a.bc[d] = g->bc[d];
     a.K = g->rho0 * g->c0 * g->c0;

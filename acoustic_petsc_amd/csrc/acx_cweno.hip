@@ -221,9 +221,10 @@ k_cweno(CwArgs p)
 
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int gx = blockIdx.x * (BX - 2) - 1 + tx;
-    const int gy = blockIdx.y * (BY - 2) - 1 + ty;
-    const int zb = blockIdx.z * p.zc, ze = min(zb + p.zc, nz);
+    const int tile = xcd_logical_tile(blockIdx.x, gridDim.x);
+    const int gx = (tile % p.ntx) * (BX - 2) - 1 + tx;
+    const int gy = ((tile / p.ntx) % p.nty) * (BY - 2) - 1 + ty;
+    const int zb = (tile / (p.ntx * p.nty)) * p.zc, ze = min(zb + p.zc, nz);
     const bool interior = tx >= 1 && tx <= BX - 2 && ty >= 1 && ty <= BY - 2 && gx < nx && gy < ny;
     const size_t plane = (size_t)nx * ny;
     const int perx = p.bc[0] == ACX_PERIODIC, pery = p.bc[1] == ACX_PERIODIC, perz = p.bc[2] == ACX_PERIODIC;
@@ -409,9 +410,13 @@ template <typename T, int SYS, int EPI, bool SEL>
 static int launch_cw_t(const CwArgs &a, hipStream_t s)
 {
     constexpr int BX = 32, BY = 16;
+    CwArgs b = a;
+    b.ntx = (a.nx + BX - 3) / (BX - 2);
+    b.nty = (a.ny + BY - 3) / (BY - 2);
+    b.ntz = (a.nz + a.zc - 1) / a.zc;
     dim3 blk(BX, BY, 1);
-    dim3 grd((a.nx + BX - 3) / (BX - 2), (a.ny + BY - 3) / (BY - 2), (a.nz + a.zc - 1) / a.zc);
-    hipLaunchKernelGGL((k_cweno<T, SYS, BX, BY, EPI, SEL>), grd, blk, 0, s, a);
+    dim3 grd(b.ntx * b.nty * b.ntz, 1, 1);
+    hipLaunchKernelGGL((k_cweno<T, SYS, BX, BY, EPI, SEL>), grd, blk, 0, s, b);
     return hip_check(hipGetLastError(), "k_cweno launch");
 }
 

@@ -34,6 +34,18 @@ __device__ __forceinline__ int map_axis(int i, int n, int periodic, float &sign)
     return r >= 0 ? r : 0;
 }
 
+// XCD-aware tile order.  Workgroups of a 1D launch are handed to the 8 XCDs
+// round-robin (id % 8); each XCD has its own L2.  Give XCD x a contiguous run
+// of logical tiles so that tiles sharing halo planes are resident on the
+// same L2 at the same time.  Bijective for any workgroup count.
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ int xcd_logical_tile(int id, int nb)
+{
+    const int xcd = id % kNumXcd, slot = id / kNumXcd;
+    const int base = nb / kNumXcd, rem = nb % kNumXcd;
+    return xcd * base + (xcd < rem ? xcd : rem) + slot;
+}
+
 template <typename T> struct Vec2 { T a, b; };
 
 // AoS cell access (4 components: p,u,v,w; or 2 components: u,v)

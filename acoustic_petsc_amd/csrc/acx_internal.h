@@ -17,6 +17,7 @@ struct LinArgs {
     void *out;         // result
     void *out2;        // F32C final stage: new LO
     int nx, ny, nz, zc;
+    int ntx, nty, ntz;   // tile counts (filled by the launcher)
     int bc[3];
     int final_stage;
     double a, b, c;    // epilogue coefficients
@@ -33,6 +34,7 @@ struct CwArgs {
     void *out2;
     const uint8_t *flags;   // selective mode: per-cell bitmask, bit v set => full CWENO for component v
     int nx, ny, nz, zc;
+    int ntx, nty, ntz;
     int bc[3];
     int final_stage;
     double a, b, c;

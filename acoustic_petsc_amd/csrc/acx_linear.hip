@@ -10,11 +10,12 @@
 // SSP stage combination  OUT = a*U0 + b*Q + c*F(Q).
 //
 // Layout: 2.5D blocking.  A block owns a BX x BY column tile and marches a
-// chunk of z-planes.  Each thread keeps its column's z-neighbourhood
-// (z-3..z+3) in registers and prefetches z+4; the current plane with an
-// x/y halo of 3 is staged in LDS (double-buffered, one barrier per plane).
-// HBM traffic per stage is one read of Q (+ U0) and one write of OUT; halo
-// and z-chunk overlap re-reads are served by L2 / Infinity Cache.
+// chunk of z-planes.  Each plane is read once: staged in LDS with an x/y
+// halo of 3 (double-buffered, one barrier per plane) for its in-plane
+// stencils, and pushed into a per-thread register queue that supplies the
+// z-stencil three planes later.  HBM traffic per stage is one read of Q
+// (+ U0) and one write of OUT; halo rows are read by neighbouring tiles at
+// the same plane index, so they meet in L2 (tiles are ordered per XCD).
 #include "acx_device.h"
 #include "acx_internal.h"
 
@@ -48,8 +49,9 @@ k_linear(LinArgs p)
 
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
-    const int zb = blockIdx.z * p.zc;
+    const int tile = xcd_logical_tile(blockIdx.x, gridDim.x);
+    const int x0 = (tile % p.ntx) * BX, y0 = ((tile / p.ntx) % p.nty) * BY;
+    const int zb = (tile / (p.ntx * p.nty)) * p.zc;
     const int ze = min(zb + p.zc, nz);
     const bool active = (x0 + tx < nx) && (y0 + ty < ny);
     const size_t plane = (size_t)nx * ny;
@@ -102,88 +104,142 @@ k_linear(LinArgs p)
         return c;
     };
 
-    // z queue: planes z-3..z+3 of the own column, plus the prefetched z+4
-    Cell<T, NV> qz[7];
-#pragma unroll
-    for (int d = -3; d <= 3; ++d) {
-        float s;
-        size_t off = zmap(zb + d, s);
-        qz[d + 3] = fetch(off + col, s * csign);
-    }
-    Cell<T, NV> hv;
-    if (hx >= 0) {
-        float s;
-        size_t off = zmap(zb, s);
-        hv = fetch(off + hcol, s * hsign);
-    }
-
+    // Lead-plane march.  At iteration j the plane j is staged in LDS (own
+    // column from the previous prefetch, halo loaded by this block at the
+    // same time as the neighbouring blocks load it as their interior, so the
+    // shared rows meet in L2), its in-plane (x, y) part of the RHS is formed
+    // and queued, and plane c = j - 3, whose z-neighbourhood c-3..c+3 is now
+    // complete in registers, is finished and stored.
+    constexpr int NP = SYS == ACX_ACOUSTIC4 ? 4 : 1;
     const T inv48h = T(p.inv48h), Kc = T(p.K), irho = T(p.invrho), sLLF = T(p.s);
     const T lap = T(p.lapc);   // WAVE2: c^2 / (48 h^2)
 
-    int buf = 0;
-    for (int z = zb; z < ze; ++z) {
-        // prefetch the next plane of the column and of the halo cell
-        Cell<T, NV> qn, hn;
-        {
-            float s;
-            size_t off = zmap(z + 4, s);
-            qn = fetch(off + col, s * csign);
-            if (hx >= 0) {
-                size_t off1 = zmap(z + 1, s);
-                hn = fetch(off1 + hcol, s * hsign);
-            }
-        }
-        // stage the current plane
+    Cell<T, NV> qz[7];   // raw column values of planes j-6 .. j after the push
+    T xyq[4][NP];        // in-plane RHS parts of planes j-3 .. j
 #pragma unroll
-        for (int v = 0; v < NS; ++v) lds[buf][v][ty + 3][tx + 3] = qz[3].v[v];
-        if (hx >= 0) {
+    for (int k = 0; k < 7; ++k)
 #pragma unroll
-            for (int v = 0; v < NS; ++v) lds[buf][v][hy][hx] = hv.v[v];
-        }
-        __syncthreads();
+        for (int v = 0; v < NV; ++v) qz[k].v[v] = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int v = 0; v < NP; ++v) xyq[k][v] = T(0);
 
-        Cell<T, NV> r;
-        if constexpr (SYS == ACX_ACOUSTIC4) {
-            T accB[4], dAp[3], dAvel = T(0);
+    Cell<T, NV> cur, hcur;
+    {
+        float s;
+        size_t off = zmap(zb - 3, s);
+        cur = fetch(off + col, s * csign);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const T *row = &lds[buf][v][ty + 3][tx + 3];
-                const T xm3 = row[-3], xm2 = row[-2], xm1 = row[-1], xp1 = row[1], xp2 = row[2], xp3 = row[3];
-                const T ym3 = lds[buf][v][ty][tx + 3], ym2 = lds[buf][v][ty + 1][tx + 3], ym1 = lds[buf][v][ty + 2][tx + 3];
-                const T yp1 = lds[buf][v][ty + 4][tx + 3], yp2 = lds[buf][v][ty + 5][tx + 3], yp3 = lds[buf][v][ty + 6][tx + 3];
-                const T q0 = qz[3].v[v];
-                accB[v] = stencil_B7(xm3, xm2, xm1, q0, xp1, xp2, xp3) +
-                          stencil_B7(ym3, ym2, ym1, q0, yp1, yp2, yp3) +
-                          stencil_B7(qz[0].v[v], qz[1].v[v], qz[2].v[v], q0, qz[4].v[v], qz[5].v[v], qz[6].v[v]);
-                if (v == 0) {
-                    dAp[0] = stencil_A7(xm3, xm2, xm1, xp1, xp2, xp3);
-                    dAp[1] = stencil_A7(ym3, ym2, ym1, yp1, yp2, yp3);
-                    dAp[2] = stencil_A7(qz[0].v[0], qz[1].v[0], qz[2].v[0], qz[4].v[0], qz[5].v[0], qz[6].v[0]);
-                } else if (v == 1) {
-                    dAvel += stencil_A7(xm3, xm2, xm1, xp1, xp2, xp3);
-                } else if (v == 2) {
-                    dAvel += stencil_A7(ym3, ym2, ym1, yp1, yp2, yp3);
-                } else {
-                    dAvel += stencil_A7(qz[0].v[3], qz[1].v[3], qz[2].v[3], qz[4].v[3], qz[5].v[3], qz[6].v[3]);
+        for (int v = 0; v < NV; ++v) hcur.v[v] = T(0);
+    }
+
+    int buf = 0;
+    for (int j = zb - 3; j <= ze + 2; ++j) {
+        const bool stage_xy = (j >= zb) && (j < ze);
+        const int c = j - 3;
+        const bool finish = (c >= zb) && (c < ze) && active;
+        // prefetch plane j+1 (own column; halo only where it will be staged)
+        Cell<T, NV> nxt, hnxt;
+        if (j + 1 <= ze + 2) {
+            float s;
+            size_t off = zmap(j + 1, s);
+            nxt = fetch(off + col, s * csign);
+            if (hx >= 0 && j + 1 >= zb && j + 1 < ze) hnxt = fetch(off + hcol, s * hsign);
+        }
+        // loads of the epilogue operands of plane c
+        Cell<T, NV> e0, e1, e2;
+        const size_t cidx = (size_t)c * plane + col;   // owned cell when finish
+        if (finish) {
+            if constexpr (EPI == EPI_COMB) e0 = load_cell<T, NV>(U0, cidx);
+            if constexpr (EPI == EPI_F32C) {
+                if (D) e0 = load_cell<T, NV>(D, cidx);
+                else {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) e0.v[v] = T(0);
+                }
+                if (p.final_stage) {
+                    e1 = load_cell<T, NV>(LO, cidx);
+                    e2 = load_cell<T, NV>(Q, cidx);   // HI of plane c, exact
                 }
             }
-            r.v[0] = inv48h * (sLLF * accB[0] - Kc * dAvel);
-            r.v[1] = inv48h * (sLLF * accB[1] - irho * dAp[0]);
-            r.v[2] = inv48h * (sLLF * accB[2] - irho * dAp[1]);
-            r.v[3] = inv48h * (sLLF * accB[3] - irho * dAp[2]);
-        } else {
-            const T *row = &lds[buf][0][ty + 3][tx + 3];
-            const T q0 = qz[3].v[0];
-            T L = stencil_L7(row[-3], row[-2], row[-1], q0, row[1], row[2], row[3]) +
-                  stencil_L7(lds[buf][0][ty][tx + 3], lds[buf][0][ty + 1][tx + 3], lds[buf][0][ty + 2][tx + 3], q0,
-                             lds[buf][0][ty + 4][tx + 3], lds[buf][0][ty + 5][tx + 3], lds[buf][0][ty + 6][tx + 3]) +
-                  stencil_L7(qz[0].v[0], qz[1].v[0], qz[2].v[0], q0, qz[4].v[0], qz[5].v[0], qz[6].v[0]);
-            r.v[0] = qz[3].v[1];     // du/dt = v (rhs_function.c:330)
-            r.v[1] = lap * L;
         }
 
-        if (active) {
-            const size_t idx = (size_t)z * plane + col;   // owned cell: no ghost mapping
+        T nxy[NP];
+#pragma unroll
+        for (int v = 0; v < NP; ++v) nxy[v] = T(0);
+        if (stage_xy) {
+            // hcur of plane j was prefetched at iteration j-1
+#pragma unroll
+            for (int v = 0; v < NS; ++v) lds[buf][v][ty + 3][tx + 3] = cur.v[v];
+            if (hx >= 0) {
+#pragma unroll
+                for (int v = 0; v < NS; ++v) lds[buf][v][hy][hx] = hcur.v[v];
+            }
+            __syncthreads();
+            if constexpr (SYS == ACX_ACOUSTIC4) {
+                T accB[4], dAx = T(0), dAy = T(0), dAvel = T(0);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const T *row = &lds[buf][v][ty + 3][tx + 3];
+                    const T q0 = cur.v[v];
+                    const T xm3 = row[-3], xm2 = row[-2], xm1 = row[-1], xp1 = row[1], xp2 = row[2], xp3 = row[3];
+                    const T ym3 = lds[buf][v][ty][tx + 3], ym2 = lds[buf][v][ty + 1][tx + 3], ym1 = lds[buf][v][ty + 2][tx + 3];
+                    const T yp1 = lds[buf][v][ty + 4][tx + 3], yp2 = lds[buf][v][ty + 5][tx + 3], yp3 = lds[buf][v][ty + 6][tx + 3];
+                    accB[v] = stencil_B7(xm3, xm2, xm1, q0, xp1, xp2, xp3) + stencil_B7(ym3, ym2, ym1, q0, yp1, yp2, yp3);
+                    if (v == 0) {
+                        dAx = stencil_A7(xm3, xm2, xm1, xp1, xp2, xp3);
+                        dAy = stencil_A7(ym3, ym2, ym1, yp1, yp2, yp3);
+                    } else if (v == 1) {
+                        dAvel += stencil_A7(xm3, xm2, xm1, xp1, xp2, xp3);
+                    } else if (v == 2) {
+                        dAvel += stencil_A7(ym3, ym2, ym1, yp1, yp2, yp3);
+                    }
+                }
+                nxy[0] = sLLF * accB[0] - Kc * dAvel;
+                nxy[1] = sLLF * accB[1] - irho * dAx;
+                nxy[2] = sLLF * accB[2] - irho * dAy;
+                nxy[3] = sLLF * accB[3];
+            } else {
+                const T *row = &lds[buf][0][ty + 3][tx + 3];
+                const T q0 = cur.v[0];
+                nxy[0] = stencil_L7(row[-3], row[-2], row[-1], q0, row[1], row[2], row[3]) +
+                         stencil_L7(lds[buf][0][ty][tx + 3], lds[buf][0][ty + 1][tx + 3], lds[buf][0][ty + 2][tx + 3], q0,
+                                    lds[buf][0][ty + 4][tx + 3], lds[buf][0][ty + 5][tx + 3], lds[buf][0][ty + 6][tx + 3]);
+            }
+            buf ^= 1;
+        }
+
+        // push plane j
+#pragma unroll
+        for (int k = 0; k < 6; ++k) qz[k] = qz[k + 1];
+        qz[6] = cur;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int v = 0; v < NP; ++v) xyq[k][v] = xyq[k + 1][v];
+#pragma unroll
+        for (int v = 0; v < NP; ++v) xyq[3][v] = nxy[v];
+
+        if (finish) {
+            // z part of plane c from qz[0..6] = planes c-3..c+3
+            Cell<T, NV> r;
+            if constexpr (SYS == ACX_ACOUSTIC4) {
+                T bz[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    bz[v] = stencil_B7(qz[0].v[v], qz[1].v[v], qz[2].v[v], qz[3].v[v], qz[4].v[v], qz[5].v[v], qz[6].v[v]);
+                const T azp = stencil_A7(qz[0].v[0], qz[1].v[0], qz[2].v[0], qz[4].v[0], qz[5].v[0], qz[6].v[0]);
+                const T azw = stencil_A7(qz[0].v[3], qz[1].v[3], qz[2].v[3], qz[4].v[3], qz[5].v[3], qz[6].v[3]);
+                r.v[0] = inv48h * (xyq[0][0] + sLLF * bz[0] - Kc * azw);
+                r.v[1] = inv48h * (xyq[0][1] + sLLF * bz[1]);
+                r.v[2] = inv48h * (xyq[0][2] + sLLF * bz[2]);
+                r.v[3] = inv48h * (xyq[0][3] + sLLF * bz[3] - irho * azp);
+            } else {
+                r.v[0] = qz[3].v[1];     // du/dt = v (rhs_function.c:330)
+                r.v[1] = lap * (xyq[0][0] + stencil_L7(qz[0].v[0], qz[1].v[0], qz[2].v[0], qz[3].v[0], qz[4].v[0],
+                                                       qz[5].v[0], qz[6].v[0]));
+            }
             Cell<T, NV> o;
             if constexpr (EPI == EPI_RHS) {
                 o = r;
@@ -193,43 +249,32 @@ k_linear(LinArgs p)
                 for (int v = 0; v < NV; ++v) o.v[v] = qz[3].v[v] + cc * r.v[v];
             } else if constexpr (EPI == EPI_COMB) {
                 const T aa = T(p.a), bb = T(p.b), cc = T(p.c);
-                Cell<T, NV> u0 = load_cell<T, NV>(U0, idx);
 #pragma unroll
-                for (int v = 0; v < NV; ++v) o.v[v] = aa * u0.v[v] + bb * qz[3].v[v] + cc * r.v[v];
+                for (int v = 0; v < NV; ++v) o.v[v] = aa * e0.v[v] + bb * qz[3].v[v] + cc * r.v[v];
             } else {
                 // compensated FP32: increments relative to the step's base (HI + LO)
                 const T bb = T(p.b), cc = T(p.c);
-                Cell<T, NV> dcur;
-                if (D) dcur = load_cell<T, NV>(D, idx);
-                else {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) dcur.v[v] = T(0);
-                }
-#pragma unroll
-                for (int v = 0; v < NV; ++v) o.v[v] = bb * dcur.v[v] + cc * r.v[v];
+                for (int v = 0; v < NV; ++v) o.v[v] = bb * e0.v[v] + cc * r.v[v];
                 if (p.final_stage) {
-                    Cell<T, NV> hi = load_cell<T, NV>(Q, idx), lo = load_cell<T, NV>(LO, idx), nh, nl;
+                    Cell<T, NV> nh, nl;
 #pragma unroll
                     for (int v = 0; v < NV; ++v) {
-                        float t = (float)lo.v[v] + (float)o.v[v];
-                        float s, e;
-                        two_sum((float)hi.v[v], t, s, e);
-                        nh.v[v] = s;
-                        nl.v[v] = e;
+                        const float hi = (float)e2.v[v];
+                        const float t = (float)e1.v[v] + (float)o.v[v];
+                        float sm, er;
+                        two_sum(hi, t, sm, er);
+                        nh.v[v] = sm;
+                        nl.v[v] = er;
                     }
-                    store_cell<T, NV>(OUT2, idx, nl);
+                    store_cell<T, NV>(OUT2, cidx, nl);
                     o = nh;
                 }
             }
-            store_cell<T, NV>(OUT, idx, o);
+            store_cell<T, NV>(OUT, cidx, o);
         }
-
-        // shift the queue
-#pragma unroll
-        for (int k = 0; k < 6; ++k) qz[k] = qz[k + 1];
-        qz[6] = qn;
-        hv = hn;
-        buf ^= 1;
+        cur = nxt;
+        hcur = hnxt;
     }
 }
 
@@ -239,9 +284,13 @@ template <typename T, int SYS, int EPI>
 static int launch_linear_t(const LinArgs &a, hipStream_t s)
 {
     constexpr int BX = 64, BY = 8;
+    LinArgs b = a;
+    b.ntx = (a.nx + BX - 1) / BX;
+    b.nty = (a.ny + BY - 1) / BY;
+    b.ntz = (a.nz + a.zc - 1) / a.zc;
     dim3 blk(BX, BY, 1);
-    dim3 grd((a.nx + BX - 1) / BX, (a.ny + BY - 1) / BY, (a.nz + a.zc - 1) / a.zc);
-    hipLaunchKernelGGL((k_linear<T, SYS, BX, BY, EPI>), grd, blk, 0, s, a);
+    dim3 grd(b.ntx * b.nty * b.ntz, 1, 1);
+    hipLaunchKernelGGL((k_linear<T, SYS, BX, BY, EPI>), grd, blk, 0, s, b);
     return hip_check(hipGetLastError(), "k_linear launch");
 }
 

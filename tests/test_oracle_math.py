@@ -1,0 +1,306 @@
+"""Closed-form and invariant checks that pin the oracle's restatement.
+
+Each check is derived from the mathematics, not from the oracle's formulas:
+polynomial reproduction (exact cell averages by Gauss quadrature), the
+smoothness indicator as a sum of squared derivative integrals (sympy),
+linear weights recovered on quadratic data, reflection symmetry, the
+stability polynomial and order of the SSP steppers, telescoping
+conservation, energy dissipation of the LLF flux, and 4th-order convergence
+of the acoustic operator to an exact plane wave.
+"""
+import ctypes as C
+import itertools
+import math
+
+import numpy as np
+import pytest
+import sympy as sp
+
+X, Y, Z = sp.symbols("x y z")
+R = sp.Rational
+BASIS3 = [1, X, Y, Z, X * X - R(1, 12), Y * Y - R(1, 12), Z * Z - R(1, 12), X * Y, Y * Z, Z * X,
+          X * (X * X - R(3, 20)), Y * (Y * Y - R(3, 20)), Z * (Z * Z - R(3, 20)),
+          (X * X - R(1, 12)) * Y, X * (Y * Y - R(1, 12)), (Y * Y - R(1, 12)) * Z, Y * (Z * Z - R(1, 12)),
+          (X * X - R(1, 12)) * Z, X * (Z * Z - R(1, 12)), X * Y * Z]
+HALF = R(1, 2)
+
+
+def cell_averages_3d(p):
+    """Exact cell averages of a polynomial (degree <= 5 per variable) over the
+    5x5x5 block of unit cells centred on the origin, indexed [z][y][x]."""
+    f = sp.lambdify((X, Y, Z), p, "numpy")
+    gx, gw = np.polynomial.legendre.leggauss(3)
+    gx, gw = gx / 2, gw / 2
+    W = gw[:, None, None] * gw[None, :, None] * gw[None, None, :]
+    blk = np.zeros((5, 5, 5))
+    for k in range(5):
+        for j in range(5):
+            for i in range(5):
+                A, B, Cc = np.meshgrid(i - 2 + gx, j - 2 + gx, k - 2 + gx, indexing="ij")
+                val = f(A, B, Cc)
+                blk[k, j, i] = np.sum(W * val) if np.ndim(val) else float(val)
+    return blk
+
+
+def derivative_energy(p, max_order):
+    """sum over multi-indices 1 <= |a| <= max_order of the cell integral of (D^a p)^2."""
+    tot = 0
+    for order in range(1, max_order + 1):
+        for combo in itertools.combinations_with_replacement((X, Y, Z), order):
+            d = sp.diff(p, *combo)
+            tot += sp.integrate(d ** 2, (X, -HALF, HALF), (Y, -HALF, HALF), (Z, -HALF, HALF))
+    return float(tot)
+
+
+def random_poly(rng, degree):
+    n = {1: 4, 2: 10, 3: 20}[degree]
+    c = np.zeros(20)
+    c[:n] = rng.standard_normal(n)
+    return c, sum(float(ci) * b for ci, b in zip(c, BASIS3))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_cweno3d_linear_reproduces_cubics(orc, seed):
+    c, p = random_poly(np.random.default_rng(seed), 3)
+    co, dg = orc.cweno3d(cell_averages_3d(p), (1.0, 0.0))
+    assert np.max(np.abs(co - c)) < 1e-12
+    assert dg[0] == 1.0   # wt_ratio of the shipped weights is exactly one
+
+
+def test_is_r4_is_derivative_energy(orc):
+    c, p = random_poly(np.random.default_rng(11), 3)
+    _, dg = orc.cweno3d(cell_averages_3d(p), (0.85, 0.85))
+    assert abs(dg[9] - derivative_energy(p, 3)) <= 1e-12 * abs(dg[9])
+
+
+@pytest.mark.parametrize("gamma", [(0.85, 0.85), (0.5, 0.3), (0.9, 0.1)])
+def test_quadratic_data_gives_linear_weights(orc, gamma):
+    """On quadratic data every sub-stencil is exact, all indicators agree and
+    the nonlinear weights equal the linear ones; the output is the exact
+    polynomial."""
+    c, p = random_poly(np.random.default_rng(5), 2)
+    co, dg = orc.cweno3d(cell_averages_3d(p), gamma)
+    g4 = gamma[0]
+    g3 = [(1 - gamma[0]) * gamma[1]] + [(1 - gamma[0]) * (1 - gamma[1]) / 6] * 6
+    assert abs(dg[1] - g4) < 1e-12
+    assert np.max(np.abs(np.array(dg[2:9]) - g3)) < 1e-12
+    assert abs(sum(dg[1:9]) - 1.0) < 1e-14
+    assert np.max(np.abs(co - c)) < 1e-12
+
+
+def test_weights_avoid_a_discontinuity(orc):
+    """A jump between the centre cell and its +x neighbour: the -x sided
+    stencil must dominate the +x sided one, and the 4th-order weight collapses."""
+    blk = np.zeros((5, 5, 5))
+    blk[:, :, 3:] = 1.0
+    _, dg = orc.cweno3d(blk, (0.85, 0.85))
+    w_px, w_mx = dg[3], dg[4]
+    assert w_mx > 100 * w_px
+    assert dg[1] < 1e-6
+    assert abs(sum(dg[1:9]) - 1.0) < 1e-14
+
+
+def test_mirror_symmetry(orc):
+    rng = np.random.default_rng(9)
+    blk = rng.standard_normal((5, 5, 5))
+    co, dg = orc.cweno3d(blk, (0.85, 0.85))
+    cm, dm = orc.cweno3d(blk[:, :, ::-1].copy(), (0.85, 0.85))
+    odd_x = [1, 7, 9, 10, 14, 18, 19]
+    sign = np.array([-1.0 if k in odd_x else 1.0 for k in range(20)])
+    assert np.max(np.abs(cm - sign * co)) < 1e-13 * np.max(np.abs(co))
+    assert abs(dm[3] - dg[4]) < 1e-13 and abs(dm[4] - dg[3]) < 1e-13
+
+
+def test_cweno2d_reproduces_quadratics_and_cubics(orc):
+    rng = np.random.default_rng(3)
+    b2 = [1, X, Y, X * X - R(1, 12), Y * Y - R(1, 12), X * Y, X * (X * X - R(3, 20)), Y * (Y * Y - R(3, 20)),
+          Y * (X * X - R(1, 12)), X * (Y * Y - R(1, 12))]
+    for deg, n in ((2, 6), (3, 10)):
+        c = np.zeros(10)
+        c[:n] = rng.standard_normal(n)
+        p = sum(float(ci) * b for ci, b in zip(c, b2))
+        f = sp.lambdify((X, Y), p, "numpy")
+        gx, gw = np.polynomial.legendre.leggauss(3)
+        gx, gw = gx / 2, gw / 2
+        blk = np.zeros((5, 5))
+        for j in range(5):
+            for i in range(5):
+                A, B = np.meshgrid(i - 2 + gx, j - 2 + gx, indexing="ij")
+                v = f(A, B)
+                blk[j, i] = np.sum(gw[:, None] * gw[None, :] * v) if np.ndim(v) else float(v)
+        lin = np.zeros(10)
+        orc.lib().orc_weno2d_linear(C.cast(blk.ctypes.data + 8 * 12, orc.dp), 1, 5, orc.P(lin))
+        assert np.max(np.abs(lin - c)) < 1e-12
+        if deg == 2:
+            nl = np.zeros(10)
+            orc.lib().orc_cweno2d(C.cast(blk.ctypes.data + 8 * 12, orc.dp), 1, 5, 0.85, 0.85, orc.P(nl))
+            assert np.max(np.abs(nl - c)) < 1e-12
+
+
+# ---- steppers ---------------------------------------------------------------
+
+RHS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def _linear_ode(a, b):
+    """u' = A u with A = [[a, -b], [b, a]] (eigenvalues a +- ib)."""
+    def f(ctx, t, U, F):
+        F[0] = a * U[0] - b * U[1]
+        F[1] = b * U[0] + a * U[1]
+        return 0
+    return RHS_FN(f)
+
+
+def _step_map(orc, fn, scheme, dt):
+    cols = []
+    for e in ([1.0, 0.0], [0.0, 1.0]):
+        u = np.array(e)
+        w0, w1, F = np.zeros(2), np.zeros(2), np.zeros(2)
+        if scheme == 0:
+            orc.lib().orc_ssp_rks3_step(C.cast(fn, C.c_void_p), None, 2, orc.P(u), 0.0, dt, 4,
+                                        orc.P(w0), orc.P(w1), orc.P(F))
+        else:
+            orc.lib().orc_ssp_rk104_step(C.cast(fn, C.c_void_p), None, 2, orc.P(u), 0.0, dt,
+                                         orc.P(w0), orc.P(w1), orc.P(F))
+        cols.append(u)
+    return np.array(cols).T
+
+
+def test_rks3_stability_polynomial(orc):
+    """One TSSSPRKS3 (s = 4) step of u' = lambda u multiplies by
+    R(z) = 1 + z + z^2/2 + z^3/6 + z^4/48."""
+    for a, b, dt in ((-0.3, 1.1, 0.7), (0.2, -2.0, 0.5), (-1.5, 0.0, 0.9)):
+        z = complex(a, b) * dt
+        Rz = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 48
+        M = _step_map(orc, _linear_ode(a, b), 0, dt)
+        assert abs(M[0, 0] - Rz.real) < 1e-14 and abs(M[1, 0] - Rz.imag) < 1e-14
+
+
+def test_rk104_is_fourth_order_on_linear_and_nonlinear(orc):
+    z = complex(-0.2, 0.9) * 0.1
+    M = _step_map(orc, _linear_ode(-0.2, 0.9), 1, 0.1)
+    ez = np.exp(z)
+    assert abs(complex(M[0, 0], M[1, 0]) - ez) < 2 * abs(z) ** 5
+    # nonlinear: u' = -u^2, u(0) = 1, u(t) = 1/(1+t)
+
+    def f(ctx, t, U, F):
+        F[0] = -U[0] * U[0]
+        F[1] = 0.0
+        return 0
+    fn = RHS_FN(f)
+    errs = []
+    for n in (10, 20, 40):
+        u = np.array([1.0, 0.0])
+        orc.lib().orc_ts_solve(C.cast(fn, C.c_void_p), None, 2, orc.P(u), 0.0, 2.0 / n, 2.0, 1, 0)
+        errs.append(abs(u[0] - 1.0 / 3.0))
+    assert math.log2(errs[1] / errs[2]) > 3.8
+
+
+def test_rks3_is_third_order_nonlinear(orc):
+    def f(ctx, t, U, F):
+        F[0] = -U[0] * U[0]
+        F[1] = math.cos(t)   # non-autonomous: checks the stage times 0, 1/2, 1, 1/2
+        return 0
+    fn = RHS_FN(f)
+    errs = []
+    for n in (20, 40, 80):
+        u = np.array([1.0, 0.0])
+        orc.lib().orc_ts_solve(C.cast(fn, C.c_void_p), None, 2, orc.P(u), 0.0, 2.0 / n, 2.0, 0, 4)
+        errs.append(abs(u[0] - 1.0 / 3.0) + abs(u[1] - math.sin(2.0)))
+    r = math.log2(errs[1] / errs[2])
+    assert 2.8 < r < 3.4
+
+
+def test_matchstep_lands_on_final_time(orc):
+    def f(ctx, t, U, F):
+        F[0] = 1.0
+        F[1] = 0.0
+        return 0
+    fn = RHS_FN(f)
+    u = np.array([0.0, 0.0])
+    steps = orc.lib().orc_ts_solve(C.cast(fn, C.c_void_p), None, 2, orc.P(u), 0.0, 0.3, 1.0, 0, 4)
+    assert steps == 4 and abs(u[0] - 1.0) < 1e-15
+
+
+# ---- right-hand sides -------------------------------------------------------
+
+@pytest.mark.parametrize("system,gamma", [(1, (1.0, 0.0)), (1, (0.85, 0.85)), (0, (1.0, 0.0)), (0, (0.85, 0.85))])
+def test_rhs_conserves_on_periodic_grid(orc, system, gamma):
+    n = (12, 10, 8)
+    g = orc.grid3d(n, 0.1, system, gamma)
+    U = np.random.default_rng(4).standard_normal((8, 10, 12, orc.nvar(system)))
+    F = orc.rhs3d(g, U)
+    comps = range(4) if system == 1 else [1]
+    for c in comps:
+        assert abs(F[..., c].sum()) < 1e-11 * np.abs(F[..., c]).sum()
+
+
+@pytest.mark.parametrize("system,gamma,bc", [(1, (1.0, 0.0), (0, 0, 0)), (1, (0.85, 0.85), (0, 0, 0)),
+                                             (0, (1.0, 0.0), (0, 0, 0)), (0, (0.85, 0.85), (1, 1, 1)),
+                                             (0, (1.0, 0.0), (0, 1, 1))])
+def test_streaming_equals_faithful(orc, system, gamma, bc):
+    n = (13, 11, 9)
+    g = orc.grid3d(n, 1.0 / 13, system, gamma, bc=bc)
+    U = np.random.default_rng(8).standard_normal((9, 11, 13, orc.nvar(system)))
+    a = orc.rhs3d(g, U, faithful=True)
+    b = orc.rhs3d(g, U, faithful=False)
+    assert np.linalg.norm(a - b) <= 1e-13 * np.linalg.norm(a)
+
+
+def test_llf_dissipates_acoustic_energy(orc):
+    """dE/dt = sum(p F_p / K + rho0 u.F_u) <= 0 for the LLF scheme (the
+    central part is skew-adjoint, the upwind part negative semi-definite)."""
+    n = (10, 10, 10)
+    for gamma in ((1.0, 0.0), (0.85, 0.85)):
+        g = orc.grid3d(n, 0.1, orc.ACOUSTIC4, gamma)
+        for seed in range(3):
+            U = np.random.default_rng(seed).standard_normal((10, 10, 10, 4))
+            F = orc.rhs3d(g, U)
+            dE = np.sum(U[..., 0] * F[..., 0]) + np.sum(U[..., 1:] * F[..., 1:])
+            assert dE < 0
+
+
+def test_central_wave_operator_conserves_energy_linear(orc):
+    """The WAVE2 central-flux operator with the linear reconstruction is
+    self-adjoint: sum(u * Lap_h w) = sum(w * Lap_h u)."""
+    n = (10, 10, 10)
+    g = orc.grid3d(n, 0.1, orc.WAVE2)
+    rng = np.random.default_rng(1)
+    U, W = rng.standard_normal((10, 10, 10, 2)), rng.standard_normal((10, 10, 10, 2))
+    U[..., 1] = 0
+    W[..., 1] = 0
+    LU, LW = orc.rhs3d(g, U)[..., 1], orc.rhs3d(g, W)[..., 1]
+    a, b = np.sum(W[..., 0] * LU), np.sum(U[..., 0] * LW)
+    assert abs(a - b) < 1e-12 * abs(a)
+
+
+def _acoustic_plane_wave(N, k, A, t):
+    """Exact cell averages of p = A sin(th) cos(wt), u = -(A/rho c) khat cos(th) sin(wt),
+    th = 2 pi k.x (rho0 = c0 = 1, unit periodic cube)."""
+    kk = math.sqrt(sum(v * v for v in k))
+    w = 2 * math.pi * kk
+    damp = np.prod([np.sinc(kd / N) for kd in k])
+    i = (np.arange(N) + 0.5) / N
+    Zc, Yc, Xc = np.meshgrid(i, i, i, indexing="ij")
+    th = 2 * math.pi * (k[0] * Xc + k[1] * Yc + k[2] * Zc)
+    U = np.zeros((N, N, N, 4))
+    U[..., 0] = A * np.sin(th) * math.cos(w * t) * damp
+    for d in range(3):
+        U[..., 1 + d] = -A * (k[d] / kk) * np.cos(th) * math.sin(w * t) * damp
+    return U
+
+
+@pytest.mark.slow
+def test_acoustic_operator_converges_at_fourth_order(orc):
+    """LLF + CWENO4 (shipped weights) on p = sin(2 pi (x+2y+z)): the error at
+    T = 0.05 with a small CFL drops by ~2^4 from N = 16 to N = 32."""
+    k, A, T = (1, 2, 1), 1e-3, 0.05
+    errs = []
+    for N in (16, 32):
+        g = orc.grid3d(N, 1.0 / N, orc.ACOUSTIC4, (1.0, 0.0))
+        U = _acoustic_plane_wave(N, k, A, 0.0)
+        dt = 0.05 / N
+        orc.ts_solve("orc_rhs3d_cb_stream", g, U, dt, T, 0, 4)
+        errs.append(np.linalg.norm(U - _acoustic_plane_wave(N, k, A, T)) / np.linalg.norm(U))
+    rate = math.log2(errs[0] / errs[1])
+    assert 3.6 < rate < 4.6, (errs, rate)
