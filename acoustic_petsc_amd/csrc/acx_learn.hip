@@ -1,0 +1,495 @@
+// acx_learn.hip -- learned troubled-cell indicator (include/acx_learn.h):
+// features, labels from the solver's own CWENO weights, MFMA forward,
+// fp32 backward + Adam, fused grid inference, flag dilation.
+//
+// MFMA: v_mfma_f32_16x16x32_bf16 (gfx950).  Per wavefront 16 samples are the
+// N columns; lane l holds sample n = l % 16 and the K slice 8*(l/16)..+7 of
+// the B operand; the A operand (weights) holds row l % 16 and the same K
+// slice; the accumulator holds rows 4*(l/16)..+3 of column l % 16.  The two
+// 16-row halves of a 32-wide layer therefore leave lane l with the outputs
+// {4g..4g+3, 16+4g..16+4g+3} (g = l/16), which is used as is for the next
+// layer's K slice by permuting that layer's weight columns once at packing.
+#include <math.h>
+#include <string.h>
+#include "acx_device.h"
+#include "acx_internal.h"
+#include "acx_cweno_core.h"
+#include "../../include/acx_learn.h"
+
+namespace acx {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+struct MlpPacked {
+    bf16x8 w1[2][64];   // layer 1 A operand per 16-row half and lane
+    bf16x8 w2[2][64];   // layer 2 A operand, K permuted to the layer-1 output order
+    float b1[32], b2[32], w3[32], b3;
+};
+
+// output feature held in slot j (0..7) of a lane of group g
+__host__ __device__ inline int kperm(int g, int j) { return j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4); }
+
+__device__ __forceinline__ float bf16_round(float x) { return (float)(__bf16)x; }
+
+// ---- features --------------------------------------------------------------
+
+// a[axis][k]: value at offset k-2 along the axis.  18 normalised undivided
+// differences (per axis: 4 first differences, 2 second differences) and a
+// log-amplitude term that tells the network how the differences compare
+// with the absolute eps = 1e-12 of the smoothness indicators.
+__device__ __forceinline__ void features19(const float (&a)[3][5], float (&f)[ACX_MLP_NFEAT])
+{
+    float d[3][4];
+    float S = 0.f;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            d[ax][k] = a[ax][k + 1] - a[ax][k];
+            S = fmaxf(S, fabsf(d[ax][k]));
+        }
+    const float inv = S > 0.f ? 1.f / S : 0.f;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) f[6 * ax + k] = d[ax][k] * inv;
+        f[6 * ax + 4] = (d[ax][1] - d[ax][0]) * inv;
+        f[6 * ax + 5] = (d[ax][3] - d[ax][2]) * inv;
+    }
+    f[18] = fminf(fmaxf((log10f(S + 1e-30f) + 6.f) / 6.f, -1.f), 2.f);
+}
+
+__device__ __forceinline__ void gather_arms(const float *__restrict__ U, const float *__restrict__ D, int nx, int ny,
+                                            int nz, int i, int j, int k, int v, float (&a)[3][5])
+{
+    auto W = [](int q, int n) { return q < 0 ? q + n : (q >= n ? q - n : q); };
+    auto at = [&](int x, int y, int z) {
+        const size_t idx = (((size_t)W(z, nz) * ny + W(y, ny)) * nx + W(x, nx)) * 4 + v;
+        return D ? U[idx] + D[idx] : U[idx];
+    };
+#pragma unroll
+    for (int o = -2; o <= 2; ++o) {
+        a[0][o + 2] = at(i + o, j, k);
+        a[1][o + 2] = at(i, j + o, k);
+        a[2][o + 2] = at(i, j, k + o);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_features(const float *__restrict__ U, int nx, int ny, int nz, float *__restrict__ X)
+{
+    const size_t ns = (size_t)nx * ny * nz * 4;
+    const size_t s = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const size_t cell = s / 4;
+    const int v = (int)(s % 4);
+    const int i = (int)(cell % nx), j = (int)((cell / nx) % ny), k = (int)(cell / ((size_t)nx * ny));
+    float a[3][5], f[ACX_MLP_NFEAT];
+    gather_arms(U, nullptr, nx, ny, nz, i, j, k, v, a);
+    features19(a, f);
+    float *o = X + s * ACX_MLP_IN;
+#pragma unroll
+    for (int q = 0; q < ACX_MLP_IN; ++q) o[q] = q < ACX_MLP_NFEAT ? f[q] : 0.f;
+}
+
+// ---- labels from the solver's weights -------------------------------------------
+
+__global__ void __launch_bounds__(256) k_labels(const double *__restrict__ U, CwArgs p, double thr,
+                                                uint8_t *__restrict__ labels, float *__restrict__ wt)
+{
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const size_t ncell = (size_t)nx * ny * nz;
+    const size_t cell = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (cell >= ncell) return;
+    const int i = (int)(cell % nx), j = (int)((cell / nx) % ny), k = (int)(cell / ((size_t)nx * ny));
+    auto W = [](int q, int n) { return q < 0 ? q + n : (q >= n ? q - n : q); };
+    const int dx[33] = {0, -2, -1, 1, 2, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, 1, -1, -1, 1, -1};
+    const int dy[33] = {0, 0, 0, 0, 0, -2, -1, 1, 2, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 0, 0, 0, 0, 1, 1, -1, 1, -1, 1, -1, -1};
+    const int dz[33] = {0, 0, 0, 0, 0, 0, 0, 0, 0, -2, -1, 1, 2, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 1, 1, 1, -1, 1, -1, -1, -1};
+    for (int v = 0; v < 4; ++v) {
+        double st[33];
+#pragma unroll
+        for (int q = 0; q < 33; ++q)
+            st[q] = U[(((size_t)W(k + dz[q], nz) * ny + W(j + dy[q], ny)) * nx + W(i + dx[q], nx)) * 4 + v];
+        Axis3<double> ax;
+        const double w = cweno_axes<double>(st, p, ax);
+        labels[cell * 4 + v] = fabs(w - 1.0) > thr ? 1 : 0;
+        if (wt) wt[cell * 4 + v] = (float)w;
+    }
+}
+
+// ---- MLP forward on MFMA ------------------------------------------------------
+
+__device__ __forceinline__ float mlp_wave(const MlpPacked &P, const bf16x8 &x, int lane, float (*h1out)[8],
+                                          float (*h2out)[8])
+{
+    const int g = lane >> 4;
+    floatx4 z = {0.f, 0.f, 0.f, 0.f};
+    floatx4 a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.w1[0][lane], x, z, 0, 0, 0);
+    floatx4 a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.w1[1][lane], x, z, 0, 0, 0);
+    float h[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        h[i] = tanhf(a0[i] + P.b1[4 * g + i]);
+        h[4 + i] = tanhf(a1[i] + P.b1[16 + 4 * g + i]);
+    }
+    bf16x8 hb;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) hb[q] = (__bf16)h[q];
+    if (h1out) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) (*h1out)[q] = (float)hb[q];
+    }
+    floatx4 c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.w2[0][lane], hb, z, 0, 0, 0);
+    floatx4 c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.w2[1][lane], hb, z, 0, 0, 0);
+    float acc = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float u0 = tanhf(c0[i] + P.b2[4 * g + i]);
+        const float u1 = tanhf(c1[i] + P.b2[16 + 4 * g + i]);
+        if (h2out) { (*h2out)[i] = u0; (*h2out)[4 + i] = u1; }
+        acc += P.w3[4 * g + i] * u0 + P.w3[16 + 4 * g + i] * u1;
+    }
+    acc += __shfl_xor(acc, 16, 64);
+    acc += __shfl_xor(acc, 32, 64);
+    return 1.f / (1.f + __expf(-(acc + P.b3)));
+}
+
+__global__ void __launch_bounds__(256) k_mlp_forward(const MlpPacked *__restrict__ Pg, const float *__restrict__ X,
+                                                     int n, float *__restrict__ Y, float *__restrict__ H1,
+                                                     float *__restrict__ H2)
+{
+    __shared__ MlpPacked P;
+    for (int q = threadIdx.x; q < (int)(sizeof(MlpPacked) / 4); q += blockDim.x)
+        reinterpret_cast<float *>(&P)[q] = reinterpret_cast<const float *>(Pg)[q];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4, sl = lane & 15;
+    const int nw = gridDim.x * (blockDim.x >> 6);
+    for (int base = (blockIdx.x * (blockDim.x >> 6) + wv) * 16; base < n; base += nw * 16) {
+        const int s = base + sl;
+        bf16x8 x;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = (__bf16)(s < n ? X[(size_t)s * ACX_MLP_IN + 8 * g + q] : 0.f);
+        float h1[8], h2[8];
+        const float y = mlp_wave(P, x, lane, H1 ? &h1 : nullptr, H2 ? &h2 : nullptr);
+        if (s < n) {
+            if (g == 0) Y[s] = y;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (H1) H1[(size_t)s * 32 + kperm(g, q)] = h1[q];
+                if (H2) H2[(size_t)s * 32 + kperm(g, q)] = h2[q];
+            }
+        }
+    }
+}
+
+// ---- fused grid inference -------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_indicator(const MlpPacked *__restrict__ Pg, const float *__restrict__ HI,
+                                                   const float *__restrict__ D, int nx, int ny, int nz, float thr,
+                                                   uint8_t *__restrict__ flags)
+{
+    __shared__ MlpPacked P;
+    for (int q = threadIdx.x; q < (int)(sizeof(MlpPacked) / 4); q += blockDim.x)
+        reinterpret_cast<float *>(&P)[q] = reinterpret_cast<const float *>(Pg)[q];
+    __syncthreads();
+    const size_t ncell = (size_t)nx * ny * nz;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4, sl = lane & 15;
+    const size_t nw = (size_t)gridDim.x * (blockDim.x >> 6);
+    // each wave pass: 4 cells x 4 components = 16 samples
+    for (size_t c0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + wv) * 4; c0 < ncell; c0 += nw * 4) {
+        const size_t cell = c0 + (sl >> 2);
+        const int v = sl & 3;
+        bf16x8 x;
+        if (cell < ncell) {
+            const int i = (int)(cell % nx), j = (int)((cell / nx) % ny), k = (int)(cell / ((size_t)nx * ny));
+            float a[3][5], f[ACX_MLP_NFEAT];
+            gather_arms(HI, D, nx, ny, nz, i, j, k, v, a);
+            features19(a, f);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int fi = 8 * g + q;
+                float val = 0.f;
+#pragma unroll
+                for (int r = 0; r < ACX_MLP_NFEAT; ++r) val = (r == fi) ? f[r] : val;
+                x[q] = (__bf16)val;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) x[q] = (__bf16)0.f;
+        }
+        const float y = mlp_wave(P, x, lane, nullptr, nullptr);
+        const unsigned long long mask = __ballot(y > thr);
+        if (lane < 4 && c0 + lane < ncell) flags[c0 + lane] = (uint8_t)((mask >> (4 * lane)) & 0xF);
+    }
+}
+
+// ---- backward (fp32) and Adam ------------------------------------------------------
+
+constexpr int BW_SAMPLES = 64;
+
+__global__ void __launch_bounds__(256) k_mlp_backward(const float *__restrict__ prm, const float *__restrict__ X,
+                                                      const float *__restrict__ H1, const float *__restrict__ H2,
+                                                      const float *__restrict__ Y, const uint8_t *__restrict__ T,
+                                                      int n, float pos_weight, float *__restrict__ grad)
+{
+    const float *W1 = prm, *W2 = prm + 1056, *w3 = prm + 2112;
+    float *gW1 = grad, *gb1 = grad + 1024, *gW2 = grad + 1056, *gb2 = grad + 2080, *gw3 = grad + 2112, *gb3 = grad + 2144;
+    __shared__ float sW2[32][33];
+    __shared__ float dh2[BW_SAMPLES][33], h1s[BW_SAMPLES][33], dh1[BW_SAMPLES][33], xs[BW_SAMPLES][33];
+    __shared__ float dzs[BW_SAMPLES], h2s[BW_SAMPLES][33];
+    for (int q = threadIdx.x; q < 1024; q += blockDim.x) sW2[q / 32][q % 32] = bf16_round(W2[q]);
+    const int s0 = blockIdx.x * BW_SAMPLES;
+    for (int q = threadIdx.x; q < BW_SAMPLES * 32; q += blockDim.x) {
+        const int a = q / 32, b = q % 32, s = s0 + a;
+        const bool ok = s < n;
+        h1s[a][b] = ok ? H1[(size_t)s * 32 + b] : 0.f;
+        h2s[a][b] = ok ? H2[(size_t)s * 32 + b] : 0.f;
+        xs[a][b] = ok ? bf16_round(X[(size_t)s * ACX_MLP_IN + b]) : 0.f;
+    }
+    __syncthreads();
+    if (threadIdx.x < BW_SAMPLES) {
+        const int a = threadIdx.x, s = s0 + a;
+        float dz = 0.f;
+        if (s < n) {
+            const float t = T[s] ? 1.f : 0.f, c = T[s] ? pos_weight : 1.f;
+            dz = c * (Y[s] - t) / (float)n;
+        }
+        dzs[a] = dz;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < BW_SAMPLES * 32; q += blockDim.x) {
+        const int a = q / 32, j = q % 32;
+        const float h = h2s[a][j];
+        dh2[a][j] = dzs[a] * w3[j] * (1.f - h * h);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < BW_SAMPLES * 32; q += blockDim.x) {
+        const int a = q / 32, kk = q % 32;
+        float acc = 0.f;
+#pragma unroll 8
+        for (int j = 0; j < 32; ++j) acc += dh2[a][j] * sW2[j][kk];
+        const float h = h1s[a][kk];
+        dh1[a][kk] = acc * (1.f - h * h);
+    }
+    __syncthreads();
+    // parameter gradients: sum over the block's samples, one atomic per entry
+    for (int q = threadIdx.x; q < 1024; q += blockDim.x) {
+        const int r = q / 32, c = q % 32;
+        float a1 = 0.f, a2 = 0.f;
+        for (int a = 0; a < BW_SAMPLES; ++a) {
+            a1 += dh1[a][r] * xs[a][c];
+            a2 += dh2[a][r] * h1s[a][c];
+        }
+        atomicAdd(&gW1[q], a1);
+        atomicAdd(&gW2[q], a2);
+    }
+    if (threadIdx.x < 32) {
+        const int r = threadIdx.x;
+        float b1 = 0.f, b2 = 0.f, w = 0.f;
+        for (int a = 0; a < BW_SAMPLES; ++a) {
+            b1 += dh1[a][r];
+            b2 += dh2[a][r];
+            w += dzs[a] * h2s[a][r];
+        }
+        atomicAdd(&gb1[r], b1);
+        atomicAdd(&gb2[r], b2);
+        atomicAdd(&gw3[r], w);
+    }
+    if (threadIdx.x == 32) {
+        float b = 0.f;
+        for (int a = 0; a < BW_SAMPLES; ++a) b += dzs[a];
+        atomicAdd(gb3, b);
+    }
+    (void)W1;
+}
+
+__global__ void k_fill(float *p, int n, float v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_adam(float *__restrict__ prm, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+                       int n, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi, vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    prm[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ void k_pack(const float *__restrict__ prm, MlpPacked *__restrict__ P)
+{
+    const float *W1 = prm, *b1 = prm + 1024, *W2 = prm + 1056, *b2 = prm + 2080, *w3 = prm + 2112, *b3 = prm + 2144;
+    const int t = threadIdx.x;   // 128 threads: half b = t / 64, lane = t % 64
+    const int b = t >> 6, lane = t & 63, g = lane >> 4, row = 16 * b + (lane & 15);
+    bf16x8 a1, a2;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        a1[j] = (__bf16)W1[row * 32 + 8 * g + j];
+        a2[j] = (__bf16)W2[row * 32 + kperm(g, j)];
+    }
+    P->w1[b][lane] = a1;
+    P->w2[b][lane] = a2;
+    if (t < 32) {
+        P->b1[t] = b1[t];
+        P->b2[t] = b2[t];
+        P->w3[t] = w3[t];
+    }
+    if (t == 0) P->b3 = b3[0];
+}
+
+__global__ void k_dilate(const uint8_t *__restrict__ in, uint8_t *__restrict__ out, int nx, int ny, int nz)
+{
+    const size_t ncell = (size_t)nx * ny * nz;
+    const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+    auto W = [](int q, int n) { return q < 0 ? q + n : (q >= n ? q - n : q); };
+    auto at = [&](int x, int y, int z) { return in[((size_t)W(z, nz) * ny + W(y, ny)) * nx + W(x, nx)]; };
+    out[c] = at(i, j, k) | at(i - 1, j, k) | at(i + 1, j, k) | at(i, j - 1, k) | at(i, j + 1, k) | at(i, j, k - 1) |
+             at(i, j, k + 1);
+}
+
+__global__ void k_count(const uint8_t *__restrict__ f, size_t n, unsigned long long *__restrict__ out)
+{
+    unsigned long long acc = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        acc += __popc((unsigned)f[i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+}
+
+static int check_ac4(const acx_grid *g)
+{
+    if (!g || g->system != ACX_ACOUSTIC4) return set_error(ACX_EINVAL, "learned indicator: ACOUSTIC4 grid required");
+    for (int d = 0; d < 3; ++d) {
+        if (g->n[d] < 5) return set_error(ACX_EINVAL, "learned indicator: at least 5 cells per axis");
+        if (g->bc[d] != ACX_PERIODIC) return set_error(ACX_EUNSUPPORTED, "learned indicator: periodic grids only");
+    }
+    return ACX_OK;
+}
+
+} // namespace acx
+
+using namespace acx;
+
+extern "C" {
+
+size_t acx_mlp_packed_bytes(void) { return sizeof(MlpPacked); }
+
+int acx_mlp_pack(const float *params, void *packed, void *stream)
+{
+    if (!params || !packed) return set_error(ACX_EINVAL, "mlp pack: null pointer");
+    hipLaunchKernelGGL(k_pack, dim3(1), dim3(128), 0, (hipStream_t)stream, params, (MlpPacked *)packed);
+    return hip_check(hipGetLastError(), "mlp pack");
+}
+
+int acx_features_f32(const acx_grid *g, const float *U, float *X, void *stream)
+{
+    int rc = check_ac4(g);
+    if (rc) return rc;
+    if (!U || !X) return set_error(ACX_EINVAL, "features: null pointer");
+    const size_t ns = (size_t)g->n[0] * g->n[1] * g->n[2] * 4;
+    hipLaunchKernelGGL(k_features, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, (hipStream_t)stream, U, g->n[0],
+                       g->n[1], g->n[2], X);
+    return hip_check(hipGetLastError(), "features");
+}
+
+int acx_weno_labels_f64(const acx_grid *g, const double *U, double thr, uint8_t *labels, float *wt, void *stream)
+{
+    int rc = check_ac4(g);
+    if (rc) return rc;
+    if (!U || !labels) return set_error(ACX_EINVAL, "labels: null pointer");
+    CwArgs p;
+    memset(&p, 0, sizeof(p));
+    p.nx = g->n[0]; p.ny = g->n[1]; p.nz = g->n[2];
+    p.g4 = g->gamma_hi;
+    p.g3c = (1.0 - g->gamma_hi) * g->gamma_lo;
+    p.g3s = (1. / 6.) * (1 - g->gamma_hi) * (1 - g->gamma_lo);
+    p.inv_g4 = 1.0 / g->gamma_hi;
+    const size_t nc = (size_t)g->n[0] * g->n[1] * g->n[2];
+    hipLaunchKernelGGL(k_labels, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, (hipStream_t)stream, U, p, thr,
+                       labels, wt);
+    return hip_check(hipGetLastError(), "labels");
+}
+
+int acx_mlp_forward(const void *packed, const float *X, int n, float *Y, float *H1, float *H2, void *stream)
+{
+    if (!packed || !X || !Y || n < 0) return set_error(ACX_EINVAL, "mlp forward: bad arguments");
+    if (n == 0) return ACX_OK;
+    const int waves = (n + 15) / 16;
+    const int blocks = waves / 4 + 1 < 4096 ? waves / 4 + 1 : 4096;
+    hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const MlpPacked *)packed, X, n, Y,
+                       H1, H2);
+    return hip_check(hipGetLastError(), "mlp forward");
+}
+
+int acx_mlp_backward(const float *params, const float *X, const float *H1, const float *H2, const float *Y,
+                     const uint8_t *T, int n, float pos_weight, float *grad, void *stream)
+{
+    if (!params || !X || !H1 || !H2 || !Y || !T || !grad || n <= 0) return set_error(ACX_EINVAL, "mlp backward: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_fill, dim3((ACX_MLP_NPARAM + 255) / 256), dim3(256), 0, s, grad, ACX_MLP_NPARAM, 0.f);
+    hipLaunchKernelGGL(k_mlp_backward, dim3((n + BW_SAMPLES - 1) / BW_SAMPLES), dim3(256), 0, s, params, X, H1, H2, Y,
+                       T, n, pos_weight, grad);
+    return hip_check(hipGetLastError(), "mlp backward");
+}
+
+int acx_mlp_adam(float *params, const float *grad, float *m, float *v, int step, float lr, float beta1, float beta2,
+                 float eps, void *stream)
+{
+    if (!params || !grad || !m || !v || step < 1) return set_error(ACX_EINVAL, "adam: bad arguments");
+    const float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3((ACX_MLP_NPARAM + 255) / 256), dim3(256), 0, (hipStream_t)stream, params, grad, m, v,
+                       ACX_MLP_NPARAM, lr, beta1, beta2, eps, c1, c2);
+    return hip_check(hipGetLastError(), "adam");
+}
+
+int acx_indicator_flags(const acx_grid *g, const float *HI, const float *D, const void *packed, float thr,
+                        uint8_t *flags, void *stream)
+{
+    int rc = check_ac4(g);
+    if (rc) return rc;
+    if (!HI || !packed || !flags) return set_error(ACX_EINVAL, "indicator: null pointer");
+    const size_t nc = (size_t)g->n[0] * g->n[1] * g->n[2];
+    const size_t groups = (nc + 3) / 4;   // one wave pass per 4 cells
+    size_t blocks = (groups + 3) / 4;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_indicator, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, (const MlpPacked *)packed,
+                       HI, D, g->n[0], g->n[1], g->n[2], thr, flags);
+    return hip_check(hipGetLastError(), "indicator");
+}
+
+int acx_flags_dilate(const acx_grid *g, const uint8_t *in, uint8_t *out, void *stream)
+{
+    if (!g || !in || !out || in == out) return set_error(ACX_EINVAL, "dilate: bad arguments");
+    const size_t nc = (size_t)g->n[0] * g->n[1] * g->n[2];
+    hipLaunchKernelGGL(k_dilate, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, (hipStream_t)stream, in, out, g->n[0],
+                       g->n[1], g->n[2]);
+    return hip_check(hipGetLastError(), "dilate");
+}
+
+int acx_flags_count(const uint8_t *flags, size_t ncells, long long *bits, void *stream)
+{
+    if (!flags || !bits) return set_error(ACX_EINVAL, "count: bad arguments");
+    unsigned long long *d = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = hip_check(hipMallocAsync((void **)&d, sizeof(unsigned long long), s), "count alloc");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, (float *)d, 2, 0.f);
+    hipLaunchKernelGGL(k_count, dim3(1024), dim3(256), 0, s, flags, ncells, d);
+    unsigned long long h = 0;
+    rc = hip_check(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s), "count copy");
+    if (!rc) rc = hip_check(hipStreamSynchronize(s), "count sync");
+    hipFreeAsync(d, s);
+    *bits = (long long)h;
+    return rc;
+}
+
+} // extern "C"

@@ -1,0 +1,254 @@
+"""Learned troubled-cell indicator: data from the solver's own weights,
+training with the library's MFMA forward / fp32 backward kernels, and the
+selective FP32 stepper that uses it (include/acx_learn.h).
+
+No counterpart in the reference (it always evaluates the full CWENO4 weights,
+3d/weno4/plane_wave/reconstruction.c:38-259).  The label of a (cell,
+component) is |wt_ratio - 1| > 0.05 with the nonlinear weights
+gamma = (0.85, 0.85) (Viscous-Multiphase-Fourth-Order/reconstruction.c:42-43),
+wt_ratio as in reconstruction.c:236.
+"""
+import ctypes as _C
+import math
+
+import numpy as np
+
+from . import _check, _ptr, _stream, lib, ACOUSTIC4, Grid, CompensatedState, field_noise
+
+NPARAM = 32 * 32 + 32 + 32 * 32 + 32 + 32 + 1
+NFEAT = 19
+OFF = {"w1": 0, "b1": 1024, "w2": 1056, "b2": 2080, "w3": 2112, "b3": 2144}
+
+
+def _bind():
+    L = lib()
+    if getattr(L, "_learn_bound", False):
+        return L
+    vp, i32, f32, dbl = _C.c_void_p, _C.c_int, _C.c_float, _C.c_double
+    gp = _C.c_void_p
+    for name, args in {
+        "acx_mlp_pack": [vp, vp, vp],
+        "acx_features_f32": [gp, vp, vp, vp],
+        "acx_weno_labels_f64": [gp, vp, dbl, vp, vp, vp],
+        "acx_mlp_forward": [vp, vp, i32, vp, vp, vp, vp],
+        "acx_mlp_backward": [vp, vp, vp, vp, vp, vp, i32, f32, vp, vp],
+        "acx_mlp_adam": [vp, vp, vp, vp, i32, f32, f32, f32, f32, vp],
+        "acx_indicator_flags": [gp, vp, vp, vp, f32, vp, vp],
+        "acx_flags_dilate": [gp, vp, vp, vp],
+        "acx_flags_count": [vp, _C.c_size_t, _C.POINTER(_C.c_longlong), vp],
+    }.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = i32
+    L.acx_mlp_packed_bytes.restype = _C.c_size_t
+    L._learn_bound = True
+    return L
+
+
+def _g(grid):
+    g = grid.c_struct()
+    return g, _C.cast(_C.pointer(g), _C.c_void_p)
+
+
+def init_params(seed=0):
+    """Xavier-uniform weights, zero biases (fp32 numpy, NPARAM)."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(NPARAM, np.float32)
+    lim1 = math.sqrt(6.0 / (NFEAT + 32))
+    w1 = rng.uniform(-lim1, lim1, (32, 32)).astype(np.float32)
+    w1[:, NFEAT:] = 0.0
+    p[OFF["w1"]:OFF["w1"] + 1024] = w1.ravel()
+    lim2 = math.sqrt(6.0 / 64)
+    p[OFF["w2"]:OFF["w2"] + 1024] = rng.uniform(-lim2, lim2, 1024)
+    p[OFF["w3"]:OFF["w3"] + 32] = rng.uniform(-lim2, lim2, 32)
+    return p
+
+
+def pack(params_dev, stream=None):
+    import torch
+    L = _bind()
+    buf = torch.empty(L.acx_mlp_packed_bytes(), dtype=torch.uint8, device="cuda")
+    _check(L.acx_mlp_pack(_ptr(params_dev, torch.float32), _C.c_void_p(buf.data_ptr()), _stream(stream)))
+    return buf
+
+
+def features(grid, U32, stream=None):
+    import torch
+    L = _bind()
+    X = torch.empty((grid.ncells * 4, 32), dtype=torch.float32, device="cuda")
+    g, gp = _g(grid)
+    _check(L.acx_features_f32(gp, _ptr(U32, torch.float32), _ptr(X, torch.float32), _stream(stream)))
+    return X
+
+
+def labels(grid, U64, thr=0.05, stream=None):
+    """(labels uint8 [ncells*4], wt_ratio float32 [ncells*4]) with the grid's gammas."""
+    import torch
+    L = _bind()
+    T = torch.empty(grid.ncells * 4, dtype=torch.uint8, device="cuda")
+    W = torch.empty(grid.ncells * 4, dtype=torch.float32, device="cuda")
+    g, gp = _g(grid)
+    _check(L.acx_weno_labels_f64(gp, _ptr(U64, torch.float64), thr, _C.c_void_p(T.data_ptr()),
+                                 _ptr(W, torch.float32), _stream(stream)))
+    return T, W
+
+
+def forward(packed, X, keep=False, stream=None):
+    import torch
+    L = _bind()
+    n = X.shape[0]
+    Y = torch.empty(n, dtype=torch.float32, device="cuda")
+    H1 = torch.empty((n, 32), dtype=torch.float32, device="cuda") if keep else None
+    H2 = torch.empty((n, 32), dtype=torch.float32, device="cuda") if keep else None
+    _check(L.acx_mlp_forward(_C.c_void_p(packed.data_ptr()), _ptr(X, torch.float32), n, _ptr(Y, torch.float32),
+                             _ptr(H1, torch.float32) if keep else None, _ptr(H2, torch.float32) if keep else None,
+                             _stream(stream)))
+    return (Y, H1, H2) if keep else Y
+
+
+def backward(params_dev, X, H1, H2, Y, T, pos_weight, stream=None):
+    import torch
+    L = _bind()
+    G = torch.empty(NPARAM, dtype=torch.float32, device="cuda")
+    _check(L.acx_mlp_backward(_ptr(params_dev, torch.float32), _ptr(X, torch.float32), _ptr(H1, torch.float32),
+                              _ptr(H2, torch.float32), _ptr(Y, torch.float32), _C.c_void_p(T.data_ptr()),
+                              X.shape[0], pos_weight, _ptr(G, torch.float32), _stream(stream)))
+    return G
+
+
+def steep_pulse_field(grid, seed, noise_rms=1e-4):
+    """Spherical steep-fronted pressure pulse p = A (1 - tanh((r - r0)/(1.5 h)))/2,
+    A ~ U[1e-3, 1e-2], r0 ~ U[0.05, 0.15], centre U[0,1)^3, over smooth
+    band-limited noise (|k| in [1, 4]); zero velocity; point values at cell
+    centres (numpy, (nz, ny, nx, 4))."""
+    rng = np.random.default_rng(seed)
+    nz, ny, nx, _ = grid.shape()
+    A = rng.uniform(1e-3, 1e-2)
+    r0 = rng.uniform(0.05, 0.15)
+    c = rng.uniform(0, 1, 3)
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    z = (np.arange(nz) + 0.5) / nz
+
+    def mi(a, c0):
+        d = a - c0
+        return d - np.round(d)
+    dx2 = mi(x, c[0])[None, None, :] ** 2
+    dy2 = mi(y, c[1])[None, :, None] ** 2
+    dz2 = mi(z, c[2])[:, None, None] ** 2
+    r = np.sqrt(dx2 + dy2 + dz2)
+    h = 1.0 / nx
+    U = field_noise(grid, seed + 7, 1.0, 4.0, noise_rms, 0.0, 0.05)
+    U[..., 0] += A * 0.5 * (1.0 - np.tanh((r - r0) / (1.5 * h)))
+    return U
+
+
+def make_dataset(n_snapshots=8, N=64, seed0=100, steps_between=12, thr=0.05):
+    """Solver-labelled samples from N^3 runs (one run per snapshot seed): the
+    FP64 nonlinear scheme advances a steep pulse; every `steps_between` steps
+    the state is labelled with its own CWENO weights.  Returns (X, T) on the
+    GPU: X float32 [n, 32], T uint8 [n]."""
+    import torch
+    from . import ssp_rks3_step, cfl_dt
+    grid = Grid(n=(N, N, N), h=1.0 / N, system=ACOUSTIC4, gamma_hi=0.85, gamma_lo=0.85)
+    Xs, Ts = [], []
+    per_run = 2
+    runs = (n_snapshots + per_run - 1) // per_run
+    dt = cfl_dt(grid, 0.4)
+    for r in range(runs):
+        U = torch.from_numpy(steep_pulse_field(grid, seed0 + r)).cuda()
+        work = (torch.empty_like(U), torch.empty_like(U))
+        for k in range(per_run):
+            if len(Xs) >= n_snapshots:
+                break
+            for _ in range(steps_between if k else 1):
+                ssp_rks3_step(grid, U, dt, work)
+            T, _ = labels(grid, U, thr)
+            Xs.append(features(grid, U.to(torch.float32)))
+            Ts.append(T)
+    return torch.cat(Xs), torch.cat(Ts)
+
+
+def train(X, T, epochs=8, batch=8192, lr=3e-3, seed=0, pos_weight=None, log=None):
+    """Trains the indicator with the library's kernels; returns fp32 params (GPU)."""
+    import torch
+    L = _bind()
+    n = X.shape[0]
+    p = torch.from_numpy(init_params(seed)).cuda()
+    m = torch.zeros_like(p)
+    v = torch.zeros_like(p)
+    if pos_weight is None:
+        npos = max(int(T.sum().item()), 1)
+        pos_weight = min(50.0, (n - npos) / npos)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    step = 0
+    for ep in range(epochs):
+        perm = torch.randperm(n, device="cuda", generator=gen)
+        tot = 0.0
+        for b0 in range(0, n, batch):
+            idx = perm[b0:b0 + batch]
+            xb, tb = X[idx].contiguous(), T[idx].contiguous()
+            packed = pack(p)
+            Y, H1, H2 = forward(packed, xb, keep=True)
+            G = backward(p, xb, H1, H2, Y, tb, pos_weight)
+            step += 1
+            _check(L.acx_mlp_adam(_ptr(p, torch.float32), _ptr(G, torch.float32), _ptr(m, torch.float32),
+                                  _ptr(v, torch.float32), step, lr, 0.9, 0.999, 1e-8, _stream()))
+            if log is not None:
+                with torch.no_grad():
+                    y = Y.clamp(1e-7, 1 - 1e-7)
+                    t = tb.float()
+                    c = torch.where(tb > 0, torch.full_like(t, pos_weight), torch.ones_like(t))
+                    tot += float((-(c * (t * torch.log(y) + (1 - t) * torch.log(1 - y)))).sum())
+        if log is not None:
+            log.append(tot / n)
+    return p
+
+
+def evaluate(packed, X, T, thr):
+    """(recall, precision, flagged fraction) of y > thr against labels T."""
+    Y = forward(packed, X)
+    pred = Y > thr
+    t = T > 0
+    tp = int((pred & t).sum())
+    return tp / max(int(t.sum()), 1), tp / max(int(pred.sum()), 1), float(pred.float().mean())
+
+
+class SelectiveStepper:
+    """FP32 compensated SSP-RK3 with the learned indicator: flags from the
+    MLP every `refresh` steps, dilated by one cell, full CWENO4 where set."""
+
+    def __init__(self, grid, packed, U64, thr=0.3, refresh=1, dilate=1):
+        import torch
+        self.grid, self.packed, self.thr, self.refresh, self.dilate = grid, packed, thr, refresh, dilate
+        self.state = CompensatedState(grid, U64)
+        self.flags = torch.empty(grid.ncells, dtype=torch.uint8, device="cuda")
+        self.tmp = torch.empty_like(self.flags)
+        self.nsteps = 0
+        self.flag_bits = []
+
+    def refresh_flags(self, stream=None):
+        L = _bind()
+        g, gp = _g(self.grid)
+        import torch
+        _check(L.acx_indicator_flags(gp, _ptr(self.state.hi, torch.float32), None,
+                                     _C.c_void_p(self.packed.data_ptr()), self.thr,
+                                     _C.c_void_p(self.tmp.data_ptr()), _stream(stream)))
+        src, dst = self.tmp, self.flags
+        for _ in range(self.dilate):
+            _check(L.acx_flags_dilate(gp, _C.c_void_p(src.data_ptr()), _C.c_void_p(dst.data_ptr()), _stream(stream)))
+            src, dst = dst, src
+        if src is not self.flags:
+            self.flags.copy_(src)
+
+    def step(self, dt):
+        if self.nsteps % self.refresh == 0:
+            self.refresh_flags()
+        self.state.step(dt, flags=self.flags)
+        self.nsteps += 1
+
+    def flagged_fraction(self):
+        L = _bind()
+        bits = _C.c_longlong()
+        _check(L.acx_flags_count(_C.c_void_p(self.flags.data_ptr()), self.grid.ncells, _C.byref(bits), _stream()))
+        return bits.value / (4.0 * self.grid.ncells)

@@ -1,0 +1,84 @@
+/*
+ * acx_learn.h -- learned troubled-cell indicator and selective stepping.
+ *
+ * No counterpart in the reference: acoustic_petsc always runs the full
+ * CWENO4 weights (3d/weno4/plane_wave/reconstruction.c:38-259 and its copies).  This adds a small
+ * MLP that predicts, per cell and component, whether the CWENO weights leave
+ * the linear weights (|wt_ratio - 1| > threshold, wt_ratio as in
+ * reconstruction.c:236), so smooth cells can take the linear 4th-order
+ * polynomial while full CWENO stays the fallback everywhere the network is
+ * not confident.
+ *
+ * Network: 19 inputs (18 normalised undivided differences, 6 per axis, plus
+ * a log-amplitude term), padded to 32 -> 32 tanh -> 32 tanh -> 1 sigmoid.
+ * Inference and the training forward run on MFMA (bf16 operands, fp32
+ * accumulation); the backward pass and Adam run in fp32.
+ */
+#ifndef ACX_LEARN_H
+#define ACX_LEARN_H
+
+#include "acx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ACX_MLP_IN   32   /* padded input width (19 used) */
+#define ACX_MLP_H    32
+#define ACX_MLP_NFEAT 19
+
+/* fp32 master parameters, row-major [out][in]; one contiguous block of
+ * ACX_MLP_NPARAM floats in this order: w1, b1, w2, b2, w3, b3. */
+#define ACX_MLP_NPARAM (32 * 32 + 32 + 32 * 32 + 32 + 32 + 1)
+
+/* Packs the fp32 parameters into the MFMA operand layout (device to device);
+ * `packed` needs acx_mlp_packed_bytes() bytes. */
+size_t acx_mlp_packed_bytes(void);
+int acx_mlp_pack(const float *params, void *packed, void *stream);
+
+/* Feature vectors of every (cell, component): X[(cell*4 + v)*32 + f] (fp32,
+ * zero padded).  U is an ACOUSTIC4 state (fp32, interlaced). */
+int acx_features_f32(const acx_grid *g, const float *U, float *X, void *stream);
+
+/* Training labels from the solver's own weights: for every (cell, component)
+ * wt_ratio of the CWENO4 reconstruction with the grid's gamma_hi/gamma_lo
+ * (FP64 state U), label = |wt_ratio - 1| > thr.  wt may be NULL. */
+int acx_weno_labels_f64(const acx_grid *g, const double *U, double thr, uint8_t *labels, float *wt, void *stream);
+
+/* MFMA forward of n samples X[n][32]: Y[n] = sigmoid output; H1, H2 (n x 32,
+ * may be NULL) keep the activations for the backward pass. */
+int acx_mlp_forward(const void *packed, const float *X, int n, float *Y, float *H1, float *H2, void *stream);
+
+/* Gradient of the class-weighted binary cross entropy
+ *   L = -(1/n) sum_i c_i [t_i log y_i + (1 - t_i) log(1 - y_i)],
+ *   c_i = pos_weight if t_i = 1 else 1,
+ * with respect to the parameters (grad: ACX_MLP_NPARAM floats, overwritten).
+ * Uses the bf16-rounded weights and inputs the forward pass used. */
+int acx_mlp_backward(const float *params, const float *X, const float *H1, const float *H2, const float *Y,
+                     const uint8_t *T, int n, float pos_weight, float *grad, void *stream);
+
+/* One Adam step (fp32): params -= lr * mhat / (sqrt(vhat) + eps). */
+int acx_mlp_adam(float *params, const float *grad, float *m, float *v, int step, float lr, float beta1,
+                 float beta2, float eps, void *stream);
+
+/* Fused inference over a grid: features of HI (+ D if not NULL), MLP,
+ * flag bit v of flags[cell] set when y(cell, v) > thr (full CWENO). */
+int acx_indicator_flags(const acx_grid *g, const float *HI, const float *D, const void *packed, float thr,
+                        uint8_t *flags, void *stream);
+
+/* flags_out[cell] = OR of flags_in over the cell and its 6 face neighbours. */
+int acx_flags_dilate(const acx_grid *g, const uint8_t *flags_in, uint8_t *flags_out, void *stream);
+
+/* Number of set bits over all cells (host result). */
+int acx_flags_count(const uint8_t *flags, size_t ncells, long long *bits, void *stream);
+
+/* One compensated FP32 SSP-RK3 step with the selective reconstruction:
+ * full CWENO4 (grid gammas) where the flag bit of a component is set, the
+ * linear 4th-order polynomial elsewhere. */
+int acx_ssp_rks3_step_f32c_sel(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
+                               float *LO_OUT, float *D0, float *D1, const uint8_t *flags, double dt, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,195 @@
+"""Learned troubled-cell indicator: MFMA forward and fp32 backward against a
+CPU FP64 MLP on the same bf16-rounded operands, labels against the oracle's
+CWENO4 weights, fused inference against features + forward, the selective
+stepper's limits, and what a trained network gains over the method without it.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return u.view(np.float32)
+
+
+def split(p):
+    p = np.asarray(p, dtype=np.float64)
+    return (p[:1024].reshape(32, 32), p[1024:1056], p[1056:2080].reshape(32, 32), p[2080:2112], p[2112:2144],
+            p[2144])
+
+
+def cpu_forward(p, X):
+    W1, b1, W2, b2, w3, b3 = split(p)
+    W1r, W2r = bf16(W1).astype(np.float64), bf16(W2).astype(np.float64)
+    x = bf16(X).astype(np.float64)
+    h1 = np.tanh(x @ W1r.T + b1)
+    h1r = bf16(h1).astype(np.float64)
+    h2 = np.tanh(h1r @ W2r.T + b2)
+    y = 1.0 / (1.0 + np.exp(-(h2 @ w3 + b3)))
+    return y, h1r, h2
+
+
+def test_mfma_forward_matches_cpu(acx, torch_gpu):
+    torch = torch_gpu
+    from acoustic_petsc_amd import learn
+    rng = np.random.default_rng(0)
+    p = learn.init_params(3)
+    p[1024:1056] = rng.uniform(-0.2, 0.2, 32)
+    p[2080:2112] = rng.uniform(-0.2, 0.2, 32)
+    p[2144] = 0.1
+    n = 1000   # not a multiple of 16
+    X = np.zeros((n, 32), np.float32)
+    X[:, :19] = rng.uniform(-1.5, 1.5, (n, 19))
+    print("[case] mfma forward n=1000", flush=True)
+    pd = torch.from_numpy(p).cuda()
+    packed = learn.pack(pd)
+    Y, H1, H2 = learn.forward(packed, torch.from_numpy(X).cuda(), keep=True)
+    y, h1r, h2 = cpu_forward(p, X)
+    assert np.max(np.abs(Y.cpu().numpy() - y)) < 2e-5
+    assert np.max(np.abs(H1.cpu().numpy() - h1r)) < 1e-2   # bf16 ulp where tanh rounds differently
+    assert np.max(np.abs(H2.cpu().numpy() - h2)) < 2e-5
+
+
+def test_backward_matches_cpu(acx, torch_gpu):
+    torch = torch_gpu
+    from acoustic_petsc_amd import learn
+    rng = np.random.default_rng(1)
+    p = learn.init_params(5)
+    n = 777
+    X = np.zeros((n, 32), np.float32)
+    X[:, :19] = rng.uniform(-1, 1, (n, 19))
+    T = (rng.uniform(size=n) < 0.2).astype(np.uint8)
+    pd = torch.from_numpy(p).cuda()
+    packed = learn.pack(pd)
+    Xd = torch.from_numpy(X).cuda()
+    Y, H1, H2 = learn.forward(packed, Xd, keep=True)
+    pw = 3.0
+    G = learn.backward(pd, Xd, H1, H2, Y, torch.from_numpy(T).cuda(), pw).cpu().numpy().astype(np.float64)
+    # CPU FP64 gradient on the GPU's forward quantities
+    W1, b1, W2, b2, w3, b3 = split(p)
+    y, h1r, h2 = Y.cpu().numpy().astype(np.float64), H1.cpu().numpy().astype(np.float64), H2.cpu().numpy().astype(np.float64)
+    t = T.astype(np.float64)
+    c = np.where(T > 0, pw, 1.0)
+    dz = c * (y - t) / n
+    dh2 = dz[:, None] * w3[None, :] * (1 - h2 ** 2)
+    dh1 = (dh2 @ bf16(W2).astype(np.float64)) * (1 - h1r ** 2)
+    xr = bf16(X).astype(np.float64)
+    ref = np.concatenate([(dh1.T @ xr).ravel(), dh1.sum(0), (dh2.T @ h1r).ravel(), dh2.sum(0), (dz[:, None] * h2).sum(0),
+                          [dz.sum()]])
+    err = np.linalg.norm(G - ref) / np.linalg.norm(ref)
+    print(f"[case] backward rel err {err:.2e}", flush=True)
+    assert err < 1e-5
+    # and a finite-difference check of the loss through the CPU model for one weight of each layer
+    def loss(pp):
+        yy, _, _ = cpu_forward(pp, X)
+        yy = np.clip(yy, 1e-12, 1 - 1e-12)
+        return -np.sum(c * (t * np.log(yy) + (1 - t) * np.log(1 - yy))) / n
+    for k in (2144, 2120, 2090, 1100, 1060, 5):
+        e = 1e-3
+        pp, pm = p.astype(np.float64).copy(), p.astype(np.float64).copy()
+        pp[k] += e
+        pm[k] -= e
+        fd = (loss(pp) - loss(pm)) / (2 * e)
+        assert abs(fd - G[k]) < 0.05 * abs(fd) + 2e-4, (k, fd, G[k])
+
+
+def test_labels_match_oracle_weights(acx, orc, torch_gpu):
+    torch = torch_gpu
+    from acoustic_petsc_amd import learn
+    N = 12
+    grid = acx.Grid(n=(N, N, N), h=1.0 / N, gamma_hi=0.85, gamma_lo=0.85)
+    rng = np.random.default_rng(2)
+    U = rng.standard_normal(grid.shape()) * 1e-3
+    U[..., 0] += np.where(np.arange(N)[None, None, :] < N // 2, 1e-3, 0.0)
+    T, W = learn.labels(grid, torch.from_numpy(U).cuda(), thr=0.05)
+    W = W.cpu().numpy().reshape(N, N, N, 4)
+    T = T.cpu().numpy().reshape(N, N, N, 4)
+    for (k, j, i, v) in [(0, 0, 0, 0), (3, 5, 5, 1), (6, 6, 6, 2), (11, 2, 9, 3), (4, 7, 1, 0)]:
+        blk = np.take(np.take(np.take(U[..., v], range(k - 2, k + 3), axis=0, mode="wrap"),
+                              range(j - 2, j + 3), axis=1, mode="wrap"), range(i - 2, i + 3), axis=2, mode="wrap")
+        _, dg = orc.cweno3d(blk, (0.85, 0.85))
+        assert abs(W[k, j, i, v] - dg[0]) < 1e-6 * abs(dg[0])
+        assert T[k, j, i, v] == (abs(dg[0] - 1) > 0.05)
+    assert 0 < T.mean() < 1
+
+
+def test_fused_indicator_equals_features_plus_forward(acx, torch_gpu):
+    torch = torch_gpu
+    from acoustic_petsc_amd import learn
+    N = 16
+    grid = acx.Grid(n=(N, N, N), h=1.0 / N, gamma_hi=0.85, gamma_lo=0.85)
+    U = torch.from_numpy(learn.steep_pulse_field(grid, 4)).cuda().to(torch.float32)
+    p = torch.from_numpy(learn.init_params(9)).cuda()
+    packed = learn.pack(p)
+    X = learn.features(grid, U)
+    Y = learn.forward(packed, X).cpu().numpy().reshape(-1, 4)
+    st = learn.SelectiveStepper(grid, packed, U.to(torch.float64), thr=0.5, dilate=0)
+    st.refresh_flags()
+    fl = st.flags.cpu().numpy()
+    want = ((Y > 0.5).astype(np.uint8) * np.array([1, 2, 4, 8], np.uint8)).sum(1)
+    near = np.any(np.abs(Y - 0.5) < 1e-6, axis=1)
+    assert np.all((fl == want) | near)
+
+
+def test_selective_limits(acx, torch_gpu):
+    """All flags set == full CWENO4 FP32 step; no flag set == the linear scheme."""
+    torch = torch_gpu
+    N = 24
+    gnl = acx.Grid(n=(N, N, N), h=1.0 / N, gamma_hi=0.85, gamma_lo=0.85)
+    glin = acx.Grid(n=(N, N, N), h=1.0 / N)
+    from acoustic_petsc_amd import learn
+    U = torch.from_numpy(learn.steep_pulse_field(gnl, 11)).cuda()
+    dt = acx.cfl_dt(gnl, 0.4)
+    ones = torch.full((gnl.ncells,), 15, dtype=torch.uint8, device="cuda")
+    zeros = torch.zeros_like(ones)
+    a, b = acx.CompensatedState(gnl, U), acx.CompensatedState(gnl, U)
+    c, d = acx.CompensatedState(gnl, U), acx.CompensatedState(glin, U)
+    for _ in range(3):
+        a.step(dt, flags=ones)
+        b.step(dt)   # general kernel, no flags
+        c.step(dt, flags=zeros)
+        d.step(dt)   # linear kernel
+    ra, rb = a.value64().cpu().numpy(), b.value64().cpu().numpy()
+    rc, rd = c.value64().cpu().numpy(), d.value64().cpu().numpy()
+    assert np.array_equal(ra, rb)
+    assert np.linalg.norm(rc - rd) / np.linalg.norm(rd) < 1e-6
+    assert np.linalg.norm(ra - rc) / np.linalg.norm(rc) > 1e-6   # the fronts do trigger the weights
+
+
+def test_trained_indicator_gain(acx, torch_gpu):
+    """Train on solver-labelled 32^3 snapshots (other seeds), then run a held-out
+    steep pulse: the learned stepper must skip most of the WENO work and stay
+    much closer to all-WENO than the purely linear scheme does."""
+    torch = torch_gpu
+    from acoustic_petsc_amd import learn
+    X, T = learn.make_dataset(n_snapshots=6, N=32, seed0=300)
+    print(f"[case] dataset {X.shape[0]} samples, positive {float(T.float().mean()):.3f}", flush=True)
+    p = learn.train(X, T, epochs=6, batch=4096, lr=3e-3, seed=1)
+    packed = learn.pack(p)
+    rec, prec, frac = learn.evaluate(packed, X, T, 0.3)
+    print(f"[case] train recall {rec:.3f} precision {prec:.3f} flagged {frac:.3f}", flush=True)
+    assert rec > 0.9
+    N = 32
+    g = acx.Grid(n=(N, N, N), h=1.0 / N, gamma_hi=0.85, gamma_lo=0.85)
+    glin = acx.Grid(n=(N, N, N), h=1.0 / N)
+    U0 = torch.from_numpy(learn.steep_pulse_field(g, 999)).cuda()
+    dt = acx.cfl_dt(g, 0.4)
+    ref = U0.clone()
+    w = (torch.empty_like(ref), torch.empty_like(ref))
+    sel = learn.SelectiveStepper(g, packed, U0, thr=0.3)
+    lin = acx.CompensatedState(glin, U0)
+    fracs = []
+    for _ in range(10):
+        acx.ssp_rks3_step(g, ref, dt, w)
+        sel.step(dt)
+        lin.step(dt)
+        fracs.append(sel.flagged_fraction())
+    e_sel = acx.diff_f32c(sel.state.hi, sel.state.lo, ref)[0]
+    e_lin = acx.diff_f32c(lin.hi, lin.lo, ref)[0]
+    print(f"[case] flagged {np.mean(fracs):.3f}, rel L2 vs all-WENO FP64: learned {e_sel:.2e}, linear {e_lin:.2e}",
+          flush=True)
+    assert np.mean(fracs) < 0.5
+    assert e_sel < 0.2 * e_lin
