@@ -21,17 +21,18 @@ int set_error(int code, const char *fmt, ...)
     return code;
 }
 
+int env_int(const char *name, int dflt)
+{
+    const char *s = getenv(name);
+    return (s && *s) ? atoi(s) : dflt;
+}
+
 int hip_check(hipError_t e, const char *what)
 {
     if (e == hipSuccess) return ACX_OK;
     return set_error(ACX_EHIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-static int env_int(const char *name, int dflt)
-{
-    const char *s = getenv(name);
-    return (s && *s) ? atoi(s) : dflt;
-}
 
 static int check_grid(const acx_grid *g)
 {

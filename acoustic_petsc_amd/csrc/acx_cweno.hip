@@ -18,9 +18,12 @@
 //
 // Layout: a block covers BX x BY reconstruction columns, the outer ring being
 // the face-neighbour halo of the (BX-2) x (BY-2) output tile, and marches a
-// chunk of z-planes.  x-neighbour face states and fluxes move by lane
-// shuffles inside a 32-lane row, y-neighbours through LDS, z-neighbours stay
-// in registers.  Selective mode (per-cell flags) replaces the full CWENO by
+// chunk of z-planes.  The block keeps the five planes m-2..m+2 of its
+// stencil region in an LDS ring (the next plane is loaded one iteration
+// ahead into registers and stored after the last read of the slot it
+// replaces).  x-neighbour face states and fluxes move by lane shuffles
+// inside a 32-lane row, y-neighbours through LDS, z-neighbours stay in
+// registers.  Selective mode (per-cell flags) replaces the full CWENO by
 // the linear 4th-order polynomial wherever the flag bit of a component is 0.
 #include "acx_device.h"
 #include "acx_internal.h"
@@ -67,13 +70,14 @@ __device__ __forceinline__ void face_flux(int d, const T (&L)[NS], const T (&R)[
 }
 
 // Shared-memory plan of one block (dynamic LDS):
-//   ring  [3][NS][BY+4][BX+4]  planes m-1, m, m+1 of the stencil region
+//   ring  [5][NS][BY+4][BX+4]  planes m-2 .. m+2 of the stencil region
 //   fym   [NS][BY][BX]         -y face state of each reconstruction column
 //   Fy    [NS][BY][BX]         flux through the +y face of each column
 template <typename T, int NS, int BX, int BY>
 struct CwLds {
     static constexpr int LX = BX + 4, LY = BY + 4;
-    static constexpr size_t ring = 3ull * NS * LY * LX;
+    static constexpr int NR = 5;
+    static constexpr size_t ring = (size_t)NR * NS * LY * LX;
     static constexpr size_t bytes = (ring + 2ull * NS * BY * BX) * sizeof(T);
 };
 
@@ -131,49 +135,52 @@ k_cweno(CwArgs p)
             rsg[r] = s1 * s2;
         }
     }
-    auto fetch = [&](size_t cell, float sign) -> Cell<T, NV> {
-        Cell<T, NV> c = load_cell<T, NV>(Q, cell);
-        if constexpr (EPI == EPI_F32C) {
-            if (D) {
-                Cell<T, NV> d = load_cell<T, NV>(D, cell);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) c.v[v] += d.v[v];
-            }
-        }
-        c.v[0] *= T(sign);
-        return c;
-    };
+    const bool has_d = (EPI == EPI_F32C) && D != nullptr;
     auto zoff = [&](int z, float &s) -> size_t {
         s = 1.f;
         return (size_t)map_axis(z, nz, perz, s) * plane;
     };
-    auto slot = [](int m) { return ((m % 3) + 3) % 3; };
-    auto load_plane = [&](int z, Cell<T, NV>(&buf)[PER]) {
+    auto slot = [](int m) { return ((m % L::NR) + L::NR) % L::NR; };
+    // Plane loads keep the raw words; the FP32 increment and the ghost sign
+    // are applied when the plane is stored, so no wait sits at the load.
+    struct Raw {
+        Cell<T, NV> q, d;
+        float s;
+    };
+    auto load_plane = [&](int z, Raw(&buf)[PER]) {
         float s;
         const size_t off = zoff(z, s);
 #pragma unroll
         for (int r = 0; r < PER; ++r)
-            if (tid + r * NT < NREG) buf[r] = fetch(off + rcol[r], s * rsg[r]);
+            if (tid + r * NT < NREG) {
+                buf[r].q = load_cell<T, NV>(Q, off + rcol[r]);
+                if (has_d) buf[r].d = load_cell<T, NV>(D, off + rcol[r]);
+                buf[r].s = s * rsg[r];
+            }
     };
-    auto store_plane = [&](int z, const Cell<T, NV>(&buf)[PER]) {
+    auto store_plane = [&](int z, const Raw(&buf)[PER]) {
         const int sl = slot(z);
 #pragma unroll
         for (int r = 0; r < PER; ++r) {
             const int idx = tid + r * NT;
             if (idx < NREG) {
+                Cell<T, NV> c = buf[r].q;
+                if (has_d) {
 #pragma unroll
-                for (int v = 0; v < NS; ++v) ring[sl][v][idx / LX][idx % LX] = buf[r].v[v];
+                    for (int v = 0; v < NV; ++v) c.v[v] += buf[r].d.v[v];
+                }
+                c.v[0] *= T(buf[r].s);
+#pragma unroll
+                for (int v = 0; v < NS; ++v) ring[sl][v][idx / LX][idx % LX] = c.v[v];
             }
         }
     };
 
-    // prologue: planes zb-2, zb-1, zb
-    {
-        Cell<T, NV> b0[PER];
-        for (int z = zb - 2; z <= zb; ++z) {
-            load_plane(z, b0);
-            store_plane(z, b0);
-        }
+    // prologue: planes zb-3 .. zb+1
+    for (int z = zb - 3; z <= zb + 1; ++z) {
+        Raw b0[PER];
+        load_plane(z, b0);
+        store_plane(z, b0);
     }
     __syncthreads();
 
@@ -184,17 +191,34 @@ k_cweno(CwArgs p)
     T vcen_prev = T(0);
     const int lx = tx + 2, ly = ty + 2;
 
+    uint8_t fl_next = 0xff;
+    if constexpr (SEL) {
+        float s;
+        fl_next = p.flags[zoff(zb - 1, s) + ocol];
+    }
     for (int m = zb - 1; m <= ze; ++m) {
-        // prefetch plane m+2 of the region and the own column at m-2, m+2
-        Cell<T, NV> nxt[PER];
-        if (m + 2 <= ze + 1) load_plane(m + 2, nxt);
-        float sm2, sp2;
-        const size_t om2 = zoff(m - 2, sm2), op2 = zoff(m + 2, sp2);
-        const Cell<T, NV> cm2 = fetch(om2 + ocol, sm2 * osign), cp2 = fetch(op2 + ocol, sp2 * osign);
-        uint8_t fl = 0xff;
+        // prefetch plane m+3 of the region, the flags of plane m+1 and the
+        // epilogue operands of plane m-1
+        Raw nxt[PER];
+        const bool pf = m + 3 <= ze + 2;
+        if (pf) load_plane(m + 3, nxt);
+        const uint8_t fl = fl_next;
         if constexpr (SEL) {
             float s;
-            fl = p.flags[zoff(m, s) + ocol];
+            if (m + 1 <= ze) fl_next = p.flags[zoff(m + 1, s) + ocol];
+        }
+        const bool fin = m >= zb + 1 && interior;
+        const size_t cell = (size_t)(m - 1) * plane + ocol;
+        Cell<T, NV> e0, e1, e2;
+        if (fin) {
+            if constexpr (EPI == EPI_COMB) e0 = load_cell<T, NV>(U0, cell);
+            if constexpr (EPI == EPI_F32C) {
+                if (has_d) e0 = load_cell<T, NV>(D, cell);
+                if (p.final_stage) {
+                    e1 = load_cell<T, NV>(LO, cell);
+                    e2 = load_cell<T, NV>(Q, cell);
+                }
+            }
         }
         T vcen = T(0);
         if constexpr (SYS == ACX_WAVE2) {
@@ -204,7 +228,7 @@ k_cweno(CwArgs p)
             if constexpr (EPI == EPI_F32C) { if (D) vcen += D[idx]; }
         }
 
-        const int s0 = slot(m - 1), s1 = slot(m), s2 = slot(m + 1);
+        const int sm2 = slot(m - 2), s0 = slot(m - 1), s1 = slot(m), s2 = slot(m + 1), sp2 = slot(m + 2);
         T fpx[NS], fmx[NS], fpy[NS], fmz[NS], fpz[NS];
 #pragma unroll
         for (int v = 0; v < NS; ++v) {
@@ -212,10 +236,11 @@ k_cweno(CwArgs p)
             const T(*P1)[LX] = ring[s1][v];
             const T(*P2)[LX] = ring[s2][v];
             T st[33];
+            const T zm2 = ring[sm2][v][ly][lx], zp2 = ring[sp2][v][ly][lx];
             st[0] = P1[ly][lx];
             st[1] = P1[ly][lx - 2]; st[2] = P1[ly][lx - 1]; st[3] = P1[ly][lx + 1]; st[4] = P1[ly][lx + 2];
             st[5] = P1[ly - 2][lx]; st[6] = P1[ly - 1][lx]; st[7] = P1[ly + 1][lx]; st[8] = P1[ly + 2][lx];
-            st[9] = cm2.v[v]; st[10] = P0[ly][lx]; st[11] = P2[ly][lx]; st[12] = cp2.v[v];
+            st[9] = zm2; st[10] = P0[ly][lx]; st[11] = P2[ly][lx]; st[12] = zp2;
             st[13] = P1[ly + 1][lx + 1]; st[14] = P1[ly - 1][lx + 1]; st[15] = P1[ly + 1][lx - 1]; st[16] = P1[ly - 1][lx - 1];
             // yz diagonals (y,z): ++ +- -+ --
             st[17] = P2[ly + 1][lx]; st[18] = P0[ly + 1][lx]; st[19] = P2[ly - 1][lx]; st[20] = P0[ly - 1][lx];
@@ -244,7 +269,7 @@ k_cweno(CwArgs p)
         }
         __syncthreads();   // B1: -y states visible; every read of plane m-1 is done
 
-        if (m + 2 <= ze + 1) store_plane(m + 2, nxt);   // into the slot of plane m-1
+        if (pf) store_plane(m + 3, nxt);   // into the slot of plane m-2
 
         T Fx[NS], Fy[NS];
         {
@@ -273,8 +298,7 @@ k_cweno(CwArgs p)
 #pragma unroll
             for (int v = 0; v < NS; ++v) { Lf[v] = zp_prev[v]; Rf[v] = fmz[v]; }
             face_flux<T, SYS, NS>(2, Lf, Rf, K, irho, sL, Fz);
-            if (m >= zb + 1 && interior) {
-                const size_t cell = (size_t)(m - 1) * plane + ocol;
+            if (fin) {
                 Cell<T, NV> r;
                 if constexpr (SYS == ACX_ACOUSTIC4) {
                     const T ih = T(p.inv_h);
@@ -287,30 +311,32 @@ k_cweno(CwArgs p)
                 Cell<T, NV> o;
                 if constexpr (EPI == EPI_RHS) {
                     o = r;
-                } else if constexpr (EPI == EPI_AXPY) {
-                    Cell<T, NV> q = load_cell<T, NV>(Q, cell);
+                } else if constexpr (EPI == EPI_AXPY || EPI == EPI_COMB) {
+                    // Q of the own (in-domain) cell of plane m-1 is still in the ring
+                    Cell<T, NV> q;
+                    if constexpr (NS == NV) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) o.v[v] = q.v[v] + T(p.c) * r.v[v];
-                } else if constexpr (EPI == EPI_COMB) {
-                    Cell<T, NV> q = load_cell<T, NV>(Q, cell), u0 = load_cell<T, NV>(U0, cell);
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) o.v[v] = T(p.a) * u0.v[v] + T(p.b) * q.v[v] + T(p.c) * r.v[v];
-                } else {
-                    Cell<T, NV> dcur;
-                    if (D) dcur = load_cell<T, NV>(D, cell);
-                    else {
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) dcur.v[v] = T(0);
+                        for (int v = 0; v < NV; ++v) q.v[v] = ring[s0][v][ly][lx];
+                    } else {
+                        q = load_cell<T, NV>(Q, cell);
                     }
+                    if constexpr (EPI == EPI_AXPY) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) o.v[v] = T(p.b) * dcur.v[v] + T(p.c) * r.v[v];
+                        for (int v = 0; v < NV; ++v) o.v[v] = q.v[v] + T(p.c) * r.v[v];
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) o.v[v] = T(p.a) * e0.v[v] + T(p.b) * q.v[v] + T(p.c) * r.v[v];
+                    }
+                } else {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) o.v[v] = (has_d ? T(p.b) * e0.v[v] : T(0)) + T(p.c) * r.v[v];
                     if (p.final_stage) {
-                        Cell<T, NV> hi = load_cell<T, NV>(Q, cell), lo = load_cell<T, NV>(LO, cell), nh, nl;
+                        Cell<T, NV> nh, nl;
 #pragma unroll
                         for (int v = 0; v < NV; ++v) {
-                            float t = (float)lo.v[v] + (float)o.v[v];
+                            float t = (float)e1.v[v] + (float)o.v[v];
                             float sm, er;
-                            two_sum((float)hi.v[v], t, sm, er);
+                            two_sum((float)e2.v[v], t, sm, er);
                             nh.v[v] = sm;
                             nl.v[v] = er;
                         }

@@ -27,7 +27,7 @@ constexpr int ENS_BX = 64;
 constexpr int ENS_MAXP = 256;
 
 struct EnsConfig { int by, zc; };
-static const EnsConfig kConfigs[6] = {{4, 16}, {4, 32}, {4, 64}, {8, 16}, {8, 32}, {8, 64}};
+static const EnsConfig kConfigs[6] = {{8, 16}, {8, 32}, {8, 64}, {16, 16}, {16, 32}, {16, 64}};
 
 template <int BY>
 __global__ void __launch_bounds__(ENS_BX *BY)
@@ -84,7 +84,7 @@ struct acx_ensemble {
     float *hi[2], *lo[2], *d0, *d1;
     int cur;                           // index of the current (hi, lo)
     int config;
-    int persistent_blocks[2];          // per rows-per-tile variant
+    int persistent_blocks[2];          // per rows-per-tile variant (8, 16)
     hipGraphExec_t graph;
     double graph_dt;
     int graph_config, graph_cur;
@@ -123,7 +123,7 @@ template <int BY>
 static int ens_launch(acx_ensemble *e, const LinArgs &a, hipStream_t s)
 {
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(1), 0, s, e->d_counter);
-    const int nb = e->persistent_blocks[BY == 4 ? 0 : 1];
+    const int nb = e->persistent_blocks[BY == 8 ? 0 : 1];
     hipLaunchKernelGGL(k_ensemble<BY>, dim3(nb), dim3(ENS_BX, BY), 0, s, e->d_probs, e->nprob, ens_total(e),
                        e->d_counter, a);
     return hip_check(hipGetLastError(), "ensemble launch");
@@ -148,7 +148,7 @@ static int ens_stage(acx_ensemble *e, const float *D, float *OUT, float *OUT2, d
     a.s = 1.0;
     a.inv48h = 1.0 / (48.0 * e->h);
     // bc = periodic (0) from memset
-    return kConfigs[e->config].by == 4 ? ens_launch<4>(e, a, s) : ens_launch<8>(e, a, s);
+    return kConfigs[e->config].by == 8 ? ens_launch<8>(e, a, s) : ens_launch<16>(e, a, s);
 }
 
 static int ens_step(acx_ensemble *e, double dt, hipStream_t s)
@@ -179,7 +179,7 @@ int acx_ensemble_create(int nprob, const int32_t *n, double h, acx_ensemble **ou
     e->dims.assign(n, n + 3 * nprob);
     e->ncell = 0;
     for (int p = 0; p < nprob; ++p) e->ncell += (size_t)n[3 * p] * n[3 * p + 1] * n[3 * p + 2];
-    e->config = 4;   // rows 8, z-chunk 32 until tuned
+    e->config = 1;   // rows 8, z-chunk 32 until tuned
     e->graph = nullptr;
     e->graph_stream = nullptr;
     const size_t bytes = e->ncell * 4 * sizeof(float);
@@ -196,13 +196,13 @@ int acx_ensemble_create(int nprob, const int32_t *n, double h, acx_ensemble **ou
         hipMemset(e->hi[0], 0, bytes);
         hipMemset(e->lo[0], 0, bytes);
         e->cur = 0;
-        int dev = 0, cus = 0, b4 = 0, b8 = 0;
+        int dev = 0, cus = 0, b8 = 0, b16 = 0;
         hipGetDevice(&dev);
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b4, k_ensemble<4>, ENS_BX * 4, 0);
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&b8, k_ensemble<8>, ENS_BX * 8, 0);
-        e->persistent_blocks[0] = cus * (b4 > 0 ? b4 : 1);
-        e->persistent_blocks[1] = cus * (b8 > 0 ? b8 : 1);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b16, k_ensemble<16>, ENS_BX * 16, 0);
+        e->persistent_blocks[0] = cus * (b8 > 0 ? b8 : 1);
+        e->persistent_blocks[1] = cus * (b16 > 0 ? b16 : 1);
         rc = ens_layout(e);
     }
     if (rc) {

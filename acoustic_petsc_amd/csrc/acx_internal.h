@@ -46,6 +46,8 @@ int launch_linear(const LinArgs &a, int sys, int epi, bool fp32, hipStream_t s);
 int launch_cweno(const CwArgs &a, int sys, int epi, bool fp32, bool selective, hipStream_t s);
 
 int set_error(int code, const char *fmt, ...);
+// integer tuning knob from the environment (dflt when unset or empty)
+int env_int(const char *name, int dflt);
 int hip_check(hipError_t e, const char *what);
 
 } // namespace acx

@@ -20,21 +20,21 @@
 
 namespace acx {
 
-template <typename T, int SYS, int BX, int BY, int EPI>
-__global__ void __launch_bounds__(BX *BY)
+template <typename T, int SYS, int BX, int BY, int EPI, int PF, int WPE>
+__global__ void __launch_bounds__(BX *BY) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_linear(LinArgs p)
 {
     using Tile = LinTile<T, SYS, BX, BY>;
     __shared__ T lds[2][Tile::NS][BY + 6][BX + 6];
     const int tile = xcd_logical_tile(blockIdx.x, gridDim.x);
-    linear_tile<T, SYS, BX, BY, EPI>(p, (tile % p.ntx) * BX, ((tile / p.ntx) % p.nty) * BY,
-                                     (tile / (p.ntx * p.nty)) * p.zc, lds);
+    linear_tile<T, SYS, BX, BY, EPI, PF>(p, (tile % p.ntx) * BX, ((tile / p.ntx) % p.nty) * BY,
+                                         (tile / (p.ntx * p.nty)) * p.zc, lds);
 }
 
 // ---------------------------------------------------------------------------
 
-template <typename T, int SYS, int EPI>
-static int launch_linear_t(const LinArgs &a, hipStream_t s)
+template <typename T, int SYS, int EPI, int PF, int WPE>
+static int launch_linear_v(const LinArgs &a, hipStream_t s)
 {
     constexpr int BX = 64, BY = 8;
     LinArgs b = a;
@@ -43,8 +43,23 @@ static int launch_linear_t(const LinArgs &a, hipStream_t s)
     b.ntz = (a.nz + a.zc - 1) / a.zc;
     dim3 blk(BX, BY, 1);
     dim3 grd(b.ntx * b.nty * b.ntz, 1, 1);
-    hipLaunchKernelGGL((k_linear<T, SYS, BX, BY, EPI>), grd, blk, 0, s, b);
+    hipLaunchKernelGGL((k_linear<T, SYS, BX, BY, EPI, PF, WPE>), grd, blk, 0, s, b);
     return hip_check(hipGetLastError(), "k_linear launch");
+}
+
+// Prefetch depth / occupancy variants (ACX_LIN_VARIANT): 0 = one plane ahead,
+// 2 waves per SIMD; 1 = one plane ahead, registers capped for 4 waves per SIMD;
+// 2 = two planes ahead; 3 = three planes ahead.
+template <typename T, int SYS, int EPI>
+static int launch_linear_t(const LinArgs &a, hipStream_t s)
+{
+    static const int var = env_int("ACX_LIN_VARIANT", sizeof(T) == 8 ? 2 : 1);
+    switch (var) {
+    case 0: return launch_linear_v<T, SYS, EPI, 1, 1>(a, s);
+    case 1: return launch_linear_v<T, SYS, EPI, 1, 4>(a, s);
+    case 3: return launch_linear_v<T, SYS, EPI, 3, 1>(a, s);
+    default: return launch_linear_v<T, SYS, EPI, 2, 1>(a, s);
+    }
 }
 
 int launch_linear(const LinArgs &a, int sys, int epi, bool fp32, hipStream_t s)

@@ -20,12 +20,13 @@ enum { EPI_RHS = 0, EPI_AXPY = 1, EPI_COMB = 2, EPI_F32C = 3 };
 
 // One tile: columns [x0, x0+BX) x [y0, y0+BY), planes [zb, ze) of the grid
 // described by p (shared by the per-grid kernel and the ensemble kernel).
-template <typename T, int SYS, int BX, int BY, int EPI>
+template <typename T, int SYS, int BX, int BY, int EPI, int PF = 2>
 __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, int zb,
                                             T (&lds)[2][LinTile<T, SYS, BX, BY>::NS][BY + 6][BX + 6])
 {
     using Tile = LinTile<T, SYS, BX, BY>;
     constexpr int NV = Tile::NV, NS = Tile::NS;
+    static_assert(BX * BY >= Tile::NHALO, "every halo cell of a plane needs a thread of its own");
 
     const T *__restrict__ Q = static_cast<const T *>(p.q);
     const T *__restrict__ D = static_cast<const T *>(p.d);       // F32C: increment added to Q
@@ -74,31 +75,55 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
         s = 1.f;
         return (size_t)map_axis(z, nz, p.bc[2] == ACX_PERIODIC, s) * plane;
     };
-    // value of a cell as the stencil sees it (component 0 carries the ghost sign)
-    auto fetch = [&](size_t idx, float sign) -> Cell<T, NV> {
-        Cell<T, NV> c = load_cell<T, NV>(Q, idx);
-        if constexpr (EPI == EPI_F32C) {
-            if (D) {
-                Cell<T, NV> d = load_cell<T, NV>(D, idx);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) c.v[v] += d.v[v];
-            }
-        }
-        c.v[0] *= T(sign);
-        return c;
-    };
 
     // Lead-plane march.  At iteration j the plane j is staged in LDS (own
-    // column from the previous prefetch, halo loaded by this block at the
-    // same time as the neighbouring blocks load it as their interior, so the
-    // shared rows meet in L2), its in-plane (x, y) part of the RHS is formed
-    // and queued, and plane c = j - 3, whose z-neighbourhood c-3..c+3 is now
-    // complete in registers, is finished and stored.
+    // column and halo loaded for the same plane index as the neighbouring
+    // blocks load those rows as their interior, so shared rows meet in L2),
+    // its in-plane (x, y) part of the RHS is formed and queued, and plane
+    // c = j - 3, whose z-neighbourhood c-3..c+3 is now complete in registers,
+    // is finished and stored.
+    //
+    // Loads run PF iterations ahead.  A slot holds the raw loaded words (the
+    // ghost sign and the FP32 increment are applied when the slot is
+    // consumed, so nothing waits on a load at issue time), and the loop is
+    // unrolled PF times so every slot keeps fixed registers while in flight.
     constexpr int NP = SYS == ACX_ACOUSTIC4 ? 4 : 1;
     const T inv48h = T(p.inv48h), Kc = T(p.K), irho = T(p.invrho), sLLF = T(p.s);
     const T lap = T(p.lapc);   // WAVE2: c^2 / (48 h^2)
+    const int jbeg = zb - 3, jend = ze + 2;
+    const bool has_d = (EPI == EPI_F32C) && D != nullptr;
+    const bool fin_stage = (EPI == EPI_F32C) && p.final_stage;
 
-    Cell<T, NV> qz[7];   // raw column values of planes j-6 .. j after the push
+    struct Slot {
+        Cell<T, NV> q, d, hq, hd, e0, e1, e2;
+        float zs;
+    };
+    Slot sl[PF];
+
+    auto issue = [&](Slot &S, int j) {
+        if (j > jend) return;
+        const size_t off = zmap(j, S.zs);
+        S.q = load_cell<T, NV>(Q, off + col);
+        if (has_d) S.d = load_cell<T, NV>(D, off + col);
+        if (hx >= 0 && j >= zb && j < ze) {
+            S.hq = load_cell<T, NV>(Q, off + hcol);
+            if (has_d) S.hd = load_cell<T, NV>(D, off + hcol);
+        }
+        const int c = j - 3;
+        if (c >= zb && c < ze && active) {
+            const size_t cidx = (size_t)c * plane + col;
+            if constexpr (EPI == EPI_COMB) S.e0 = load_cell<T, NV>(U0, cidx);
+            if constexpr (EPI == EPI_F32C) {
+                if (has_d) S.e0 = load_cell<T, NV>(D, cidx);
+                if (fin_stage) {
+                    S.e1 = load_cell<T, NV>(LO, cidx);
+                    S.e2 = load_cell<T, NV>(Q, cidx);   // HI of plane c, exact
+                }
+            }
+        }
+    };
+
+    Cell<T, NV> qz[7];   // column values of planes j-6 .. j after the push
     T xyq[4][NP];        // in-plane RHS parts of planes j-3 .. j
 #pragma unroll
     for (int k = 0; k < 7; ++k)
@@ -109,156 +134,148 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
 #pragma unroll
         for (int v = 0; v < NP; ++v) xyq[k][v] = T(0);
 
-    Cell<T, NV> cur, hcur;
-    {
-        float s;
-        size_t off = zmap(zb - 3, s);
-        cur = fetch(off + col, s * csign);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) hcur.v[v] = T(0);
-    }
+    for (int u = 0; u < PF; ++u) issue(sl[u], jbeg + u);
 
     int buf = 0;
-    for (int j = zb - 3; j <= ze + 2; ++j) {
-        const bool stage_xy = (j >= zb) && (j < ze);
-        const int c = j - 3;
-        const bool finish = (c >= zb) && (c < ze) && active;
-        // prefetch plane j+1 (own column; halo only where it will be staged)
-        Cell<T, NV> nxt, hnxt;
-        if (j + 1 <= ze + 2) {
-            float s;
-            size_t off = zmap(j + 1, s);
-            nxt = fetch(off + col, s * csign);
-            if (hx >= 0 && j + 1 >= zb && j + 1 < ze) hnxt = fetch(off + hcol, s * hsign);
-        }
-        // loads of the epilogue operands of plane c
-        Cell<T, NV> e0, e1, e2;
-        const size_t cidx = (size_t)c * plane + col;   // owned cell when finish
-        if (finish) {
-            if constexpr (EPI == EPI_COMB) e0 = load_cell<T, NV>(U0, cidx);
-            if constexpr (EPI == EPI_F32C) {
-                if (D) e0 = load_cell<T, NV>(D, cidx);
-                else {
+    for (int jb = jbeg; jb <= jend; jb += PF) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) e0.v[v] = T(0);
-                }
-                if (p.final_stage) {
-                    e1 = load_cell<T, NV>(LO, cidx);
-                    e2 = load_cell<T, NV>(Q, cidx);   // HI of plane c, exact
-                }
-            }
-        }
+        for (int u = 0; u < PF; ++u) {
+            const int j = jb + u;
+            if (j > jend) break;
+            Slot &S = sl[u];
+            const bool stage_xy = (j >= zb) && (j < ze);
+            const int c = j - 3;
+            const bool finish = (c >= zb) && (c < ze) && active;
 
-        T nxy[NP];
+            // consume the slot of plane j
+            Cell<T, NV> cur = S.q;
+            if (has_d) {
 #pragma unroll
-        for (int v = 0; v < NP; ++v) nxy[v] = T(0);
-        if (stage_xy) {
-            // hcur of plane j was prefetched at iteration j-1
-#pragma unroll
-            for (int v = 0; v < NS; ++v) lds[buf][v][ty + 3][tx + 3] = cur.v[v];
-            if (hx >= 0) {
-#pragma unroll
-                for (int v = 0; v < NS; ++v) lds[buf][v][hy][hx] = hcur.v[v];
+                for (int v = 0; v < NV; ++v) cur.v[v] += S.d.v[v];
             }
-            __syncthreads();
-            if constexpr (SYS == ACX_ACOUSTIC4) {
-                T accB[4], dAx = T(0), dAy = T(0), dAvel = T(0);
+            cur.v[0] *= T(S.zs * csign);
+
+            T nxy[NP];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const T *row = &lds[buf][v][ty + 3][tx + 3];
-                    const T q0 = cur.v[v];
-                    const T xm3 = row[-3], xm2 = row[-2], xm1 = row[-1], xp1 = row[1], xp2 = row[2], xp3 = row[3];
-                    const T ym3 = lds[buf][v][ty][tx + 3], ym2 = lds[buf][v][ty + 1][tx + 3], ym1 = lds[buf][v][ty + 2][tx + 3];
-                    const T yp1 = lds[buf][v][ty + 4][tx + 3], yp2 = lds[buf][v][ty + 5][tx + 3], yp3 = lds[buf][v][ty + 6][tx + 3];
-                    accB[v] = stencil_B7(xm3, xm2, xm1, q0, xp1, xp2, xp3) + stencil_B7(ym3, ym2, ym1, q0, yp1, yp2, yp3);
-                    if (v == 0) {
-                        dAx = stencil_A7(xm3, xm2, xm1, xp1, xp2, xp3);
-                        dAy = stencil_A7(ym3, ym2, ym1, yp1, yp2, yp3);
-                    } else if (v == 1) {
-                        dAvel += stencil_A7(xm3, xm2, xm1, xp1, xp2, xp3);
-                    } else if (v == 2) {
-                        dAvel += stencil_A7(ym3, ym2, ym1, yp1, yp2, yp3);
+            for (int v = 0; v < NP; ++v) nxy[v] = T(0);
+            if (stage_xy) {
+#pragma unroll
+                for (int v = 0; v < NS; ++v) lds[buf][v][ty + 3][tx + 3] = cur.v[v];
+                if (hx >= 0) {
+                    Cell<T, NV> hcur = S.hq;
+                    if (has_d) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) hcur.v[v] += S.hd.v[v];
+                    }
+                    hcur.v[0] *= T(S.zs * hsign);
+#pragma unroll
+                    for (int v = 0; v < NS; ++v) lds[buf][v][hy][hx] = hcur.v[v];
+                }
+                __syncthreads();
+                if constexpr (SYS == ACX_ACOUSTIC4) {
+                    T accB[4], dAx = T(0), dAy = T(0), dAvel = T(0);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const T *row = &lds[buf][v][ty + 3][tx + 3];
+                        const T q0 = cur.v[v];
+                        const T xm3 = row[-3], xm2 = row[-2], xm1 = row[-1], xp1 = row[1], xp2 = row[2], xp3 = row[3];
+                        const T ym3 = lds[buf][v][ty][tx + 3], ym2 = lds[buf][v][ty + 1][tx + 3];
+                        const T ym1 = lds[buf][v][ty + 2][tx + 3];
+                        const T yp1 = lds[buf][v][ty + 4][tx + 3], yp2 = lds[buf][v][ty + 5][tx + 3];
+                        const T yp3 = lds[buf][v][ty + 6][tx + 3];
+                        accB[v] = stencil_B7(xm3, xm2, xm1, q0, xp1, xp2, xp3) + stencil_B7(ym3, ym2, ym1, q0, yp1, yp2, yp3);
+                        if (v == 0) {
+                            dAx = stencil_A7(xm3, xm2, xm1, xp1, xp2, xp3);
+                            dAy = stencil_A7(ym3, ym2, ym1, yp1, yp2, yp3);
+                        } else if (v == 1) {
+                            dAvel += stencil_A7(xm3, xm2, xm1, xp1, xp2, xp3);
+                        } else if (v == 2) {
+                            dAvel += stencil_A7(ym3, ym2, ym1, yp1, yp2, yp3);
+                        }
+                    }
+                    nxy[0] = sLLF * accB[0] - Kc * dAvel;
+                    nxy[1] = sLLF * accB[1] - irho * dAx;
+                    nxy[2] = sLLF * accB[2] - irho * dAy;
+                    nxy[3] = sLLF * accB[3];
+                } else {
+                    const T *row = &lds[buf][0][ty + 3][tx + 3];
+                    const T q0 = cur.v[0];
+                    nxy[0] = stencil_L7(row[-3], row[-2], row[-1], q0, row[1], row[2], row[3]) +
+                             stencil_L7(lds[buf][0][ty][tx + 3], lds[buf][0][ty + 1][tx + 3], lds[buf][0][ty + 2][tx + 3],
+                                        q0, lds[buf][0][ty + 4][tx + 3], lds[buf][0][ty + 5][tx + 3],
+                                        lds[buf][0][ty + 6][tx + 3]);
+                }
+                buf ^= 1;
+            }
+
+            // push plane j
+#pragma unroll
+            for (int k = 0; k < 6; ++k) qz[k] = qz[k + 1];
+            qz[6] = cur;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int v = 0; v < NP; ++v) xyq[k][v] = xyq[k + 1][v];
+#pragma unroll
+            for (int v = 0; v < NP; ++v) xyq[3][v] = nxy[v];
+
+            if (finish) {
+                // z part of plane c from qz[0..6] = planes c-3..c+3
+                const size_t cidx = (size_t)c * plane + col;
+                Cell<T, NV> r;
+                if constexpr (SYS == ACX_ACOUSTIC4) {
+                    T bz[4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        bz[v] = stencil_B7(qz[0].v[v], qz[1].v[v], qz[2].v[v], qz[3].v[v], qz[4].v[v], qz[5].v[v],
+                                           qz[6].v[v]);
+                    const T azp = stencil_A7(qz[0].v[0], qz[1].v[0], qz[2].v[0], qz[4].v[0], qz[5].v[0], qz[6].v[0]);
+                    const T azw = stencil_A7(qz[0].v[3], qz[1].v[3], qz[2].v[3], qz[4].v[3], qz[5].v[3], qz[6].v[3]);
+                    r.v[0] = inv48h * (xyq[0][0] + sLLF * bz[0] - Kc * azw);
+                    r.v[1] = inv48h * (xyq[0][1] + sLLF * bz[1]);
+                    r.v[2] = inv48h * (xyq[0][2] + sLLF * bz[2]);
+                    r.v[3] = inv48h * (xyq[0][3] + sLLF * bz[3] - irho * azp);
+                } else {
+                    r.v[0] = qz[3].v[1];     // du/dt = v (rhs_function.c:330)
+                    r.v[1] = lap * (xyq[0][0] + stencil_L7(qz[0].v[0], qz[1].v[0], qz[2].v[0], qz[3].v[0], qz[4].v[0],
+                                                           qz[5].v[0], qz[6].v[0]));
+                }
+                Cell<T, NV> o;
+                if constexpr (EPI == EPI_RHS) {
+                    o = r;
+                } else if constexpr (EPI == EPI_AXPY) {
+                    const T cc = T(p.c);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) o.v[v] = qz[3].v[v] + cc * r.v[v];
+                } else if constexpr (EPI == EPI_COMB) {
+                    const T aa = T(p.a), bb = T(p.b), cc = T(p.c);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) o.v[v] = aa * S.e0.v[v] + bb * qz[3].v[v] + cc * r.v[v];
+                } else {
+                    // compensated FP32: increments relative to the step's base (HI + LO)
+                    const T bb = T(p.b), cc = T(p.c);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) o.v[v] = (has_d ? bb * S.e0.v[v] : T(0)) + cc * r.v[v];
+                    if (fin_stage) {
+                        Cell<T, NV> nh, nl;
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) {
+                            const float hi = (float)S.e2.v[v];
+                            const float t = (float)S.e1.v[v] + (float)o.v[v];
+                            float sm, er;
+                            two_sum(hi, t, sm, er);
+                            nh.v[v] = sm;
+                            nl.v[v] = er;
+                        }
+                        store_cell<T, NV>(OUT2, cidx, nl);
+                        o = nh;
                     }
                 }
-                nxy[0] = sLLF * accB[0] - Kc * dAvel;
-                nxy[1] = sLLF * accB[1] - irho * dAx;
-                nxy[2] = sLLF * accB[2] - irho * dAy;
-                nxy[3] = sLLF * accB[3];
-            } else {
-                const T *row = &lds[buf][0][ty + 3][tx + 3];
-                const T q0 = cur.v[0];
-                nxy[0] = stencil_L7(row[-3], row[-2], row[-1], q0, row[1], row[2], row[3]) +
-                         stencil_L7(lds[buf][0][ty][tx + 3], lds[buf][0][ty + 1][tx + 3], lds[buf][0][ty + 2][tx + 3], q0,
-                                    lds[buf][0][ty + 4][tx + 3], lds[buf][0][ty + 5][tx + 3], lds[buf][0][ty + 6][tx + 3]);
+                store_cell<T, NV>(OUT, cidx, o);
             }
-            buf ^= 1;
+            issue(S, j + PF);   // refill this slot for plane j + PF
         }
-
-        // push plane j
-#pragma unroll
-        for (int k = 0; k < 6; ++k) qz[k] = qz[k + 1];
-        qz[6] = cur;
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int v = 0; v < NP; ++v) xyq[k][v] = xyq[k + 1][v];
-#pragma unroll
-        for (int v = 0; v < NP; ++v) xyq[3][v] = nxy[v];
-
-        if (finish) {
-            // z part of plane c from qz[0..6] = planes c-3..c+3
-            Cell<T, NV> r;
-            if constexpr (SYS == ACX_ACOUSTIC4) {
-                T bz[4];
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    bz[v] = stencil_B7(qz[0].v[v], qz[1].v[v], qz[2].v[v], qz[3].v[v], qz[4].v[v], qz[5].v[v], qz[6].v[v]);
-                const T azp = stencil_A7(qz[0].v[0], qz[1].v[0], qz[2].v[0], qz[4].v[0], qz[5].v[0], qz[6].v[0]);
-                const T azw = stencil_A7(qz[0].v[3], qz[1].v[3], qz[2].v[3], qz[4].v[3], qz[5].v[3], qz[6].v[3]);
-                r.v[0] = inv48h * (xyq[0][0] + sLLF * bz[0] - Kc * azw);
-                r.v[1] = inv48h * (xyq[0][1] + sLLF * bz[1]);
-                r.v[2] = inv48h * (xyq[0][2] + sLLF * bz[2]);
-                r.v[3] = inv48h * (xyq[0][3] + sLLF * bz[3] - irho * azp);
-            } else {
-                r.v[0] = qz[3].v[1];     // du/dt = v (rhs_function.c:330)
-                r.v[1] = lap * (xyq[0][0] + stencil_L7(qz[0].v[0], qz[1].v[0], qz[2].v[0], qz[3].v[0], qz[4].v[0],
-                                                       qz[5].v[0], qz[6].v[0]));
-            }
-            Cell<T, NV> o;
-            if constexpr (EPI == EPI_RHS) {
-                o = r;
-            } else if constexpr (EPI == EPI_AXPY) {
-                const T cc = T(p.c);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) o.v[v] = qz[3].v[v] + cc * r.v[v];
-            } else if constexpr (EPI == EPI_COMB) {
-                const T aa = T(p.a), bb = T(p.b), cc = T(p.c);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) o.v[v] = aa * e0.v[v] + bb * qz[3].v[v] + cc * r.v[v];
-            } else {
-                // compensated FP32: increments relative to the step's base (HI + LO)
-                const T bb = T(p.b), cc = T(p.c);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) o.v[v] = bb * e0.v[v] + cc * r.v[v];
-                if (p.final_stage) {
-                    Cell<T, NV> nh, nl;
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) {
-                        const float hi = (float)e2.v[v];
-                        const float t = (float)e1.v[v] + (float)o.v[v];
-                        float sm, er;
-                        two_sum(hi, t, sm, er);
-                        nh.v[v] = sm;
-                        nl.v[v] = er;
-                    }
-                    store_cell<T, NV>(OUT2, cidx, nl);
-                    o = nh;
-                }
-            }
-            store_cell<T, NV>(OUT, cidx, o);
-        }
-        cur = nxt;
-        hcur = hnxt;
     }
 }
 

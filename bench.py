@@ -52,7 +52,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--variants", default="all", choices=["all", "none"])
+    ap.add_argument("--variants", default="all",
+                    help="all, none, or a comma list of fp64_cweno, fp32c, learned, ensemble")
+    ap.add_argument("--learned-sizes", default="256,384",
+                    type=lambda s: [int(v) for v in s.split(",") if v])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain device copy of the state (counter calibration)")
@@ -115,6 +118,182 @@ def cpu_baseline(acx, n_sample=96, max_steps=8, budget_s=10.0):
                       f"({STAGES * steps} RHS) of the oracle's reference-structured RHS, {el:.1f} s"}
 
 
+VARIANTS = ("fp64_cweno", "fp32c", "learned", "ensemble")
+
+
+def variant_fp64_cweno(torch, acx, N, U0, work, dt, steps):
+    """Nonlinear CWENO4 (gamma = 0.85/0.85) FP64 on the headline workload."""
+    g = acx.Grid(n=(N, N, N), h=1.0 / N, system=acx.ACOUSTIC4, gamma_hi=0.85, gamma_lo=0.85)
+    U = torch.from_numpy(U0).cuda()
+    k = max(2, min(10, steps))
+    t, _ = time_steps(torch, lambda: acx.ssp_rks3_step(g, U, dt, work), k, 1)
+    v = g.ncells * STAGES * k / t
+    return {"value": v, "ms_per_step": 1e3 * t / k, "steps": k,
+            "hbm_frac": v * BYTES_FP64_AC4 / 1e9 / HBM_PEAK_GBS}
+
+
+def variant_fp32c(torch, acx, M, steps, fp64_value, err_steps=200):
+    """FP32 storage + compensated accumulation, shipped linear weights, M^3,
+    band-limited noise (|k| in [4, 16], RMS 1e-4) + one Gaussian pulse; error
+    against the FP64 stepper (equal to the FP64 oracle to 1e-12) after
+    `err_steps` steps."""
+    g = acx.Grid(n=(M, M, M), h=1.0 / M, system=acx.ACOUSTIC4)
+    V0 = acx.field_noise(g, 3030, 4.0, 16.0, 1e-4, 1e-3, 0.05)
+    dt = acx.cfl_dt(g, 0.4)
+    st = acx.CompensatedState(g, torch.from_numpy(V0).cuda())
+    k = max(2, min(20, steps))
+    t, _ = time_steps(torch, lambda: st.step(dt), k, 1)
+    v = M ** 3 * STAGES * k / t
+    del st
+    ref = torch.from_numpy(V0).cuda()
+    w = (torch.empty_like(ref), torch.empty_like(ref))
+    st = acx.CompensatedState(g, ref)
+    for _ in range(err_steps):
+        acx.ssp_rks3_step(g, ref, dt, w)
+        st.step(dt)
+    rel, mx = acx.diff_f32c(st.hi, st.lo, ref)
+    out = {"value": v, "ms_per_step": 1e3 * t / k, "steps": k, "grid": [M, M, M],
+           "bytes_per_cell_update": f32c_bytes_per_update(),
+           "hbm_frac": v * f32c_bytes_per_update() / 1e9 / HBM_PEAK_GBS,
+           "error_steps": err_steps, "rel_l2_vs_fp64": rel, "max_abs_vs_fp64": mx,
+           "max_abs_state": float(ref.abs().max()),
+           "speedup_vs_fp64_linear_256": v / fp64_value}
+    del st, ref, w
+    torch.cuda.empty_cache()
+    return out
+
+
+def train_indicator(torch, acx):
+    """The troubled-cell MLP trained on the GPU by the library's own kernels:
+    8 solver-labelled snapshots of 64^3 steep-pulse runs (2.1e6 cells, 8.4e6
+    (cell, component) samples), label |wt_ratio - 1| > 0.05."""
+    from acoustic_petsc_amd import learn
+    t0 = time.perf_counter()
+    X, T = learn.make_dataset(n_snapshots=8, N=64, seed0=100)
+    p = learn.train(X, T, epochs=6, batch=8192, lr=3e-3, seed=1)
+    packed = learn.pack(p)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    rec, prec, frac = learn.evaluate(packed, X, T, 0.3)
+    info = {"samples": int(X.shape[0]), "positive_frac": float(T.float().mean()), "train_s": el,
+            "recall": rec, "precision": prec, "flagged_frac": frac}
+    del X, T
+    return packed, info
+
+
+def variant_learned(torch, acx, sizes, steps, var, fp64_value):
+    """FP32 compensated stepper with the learned indicator (full CWENO4 on
+    flagged cells, linear reconstruction elsewhere) against the FP64 nonlinear
+    stepper on a held-out steep pulse, nonlinear weights gamma = 0.85/0.85."""
+    from acoustic_petsc_amd import learn
+    packed, info = train_indicator(torch, acx)
+    res = {"indicator_training": info}
+    for M in sizes:
+        g = acx.Grid(n=(M, M, M), h=1.0 / M, system=acx.ACOUSTIC4, gamma_hi=0.85, gamma_lo=0.85)
+        glin = acx.Grid(n=(M, M, M), h=1.0 / M, system=acx.ACOUSTIC4)
+        U0 = torch.from_numpy(learn.steep_pulse_field(g, 4242))
+        dt = acx.cfl_dt(g, 0.4)
+        k = max(2, min(20, steps))
+        cells = M ** 3
+        # FP64 all-WENO: the error reference and the plain FP64 throughput at M^3
+        ref = U0.cuda()
+        w = (torch.empty_like(ref), torch.empty_like(ref))
+        acx.ssp_rks3_step(g, ref, dt, w)     # warm-up step, mirrored below
+        t64, _ = time_steps(torch, lambda: acx.ssp_rks3_step(g, ref, dt, w), k, 0)
+        lin64 = U0.cuda()
+        tl64, _ = time_steps(torch, lambda: acx.ssp_rks3_step(glin, lin64, dt, w), k, 1)
+        del lin64, w
+        sel = learn.SelectiveStepper(g, packed, U0.cuda(), thr=0.3)
+        sel.step(dt)
+        fr = []
+        tsel, _ = time_steps(torch, lambda: sel.step(dt), k, 0)
+        fr.append(sel.flagged_fraction())
+        rel, mx = acx.diff_f32c(sel.state.hi, sel.state.lo, ref)
+        del sel
+        lin = acx.CompensatedState(glin, U0.cuda())
+        for _ in range(k + 1):
+            lin.step(dt)
+        rel_lin, _ = acx.diff_f32c(lin.hi, lin.lo, ref)
+        del lin
+        full = acx.CompensatedState(g, U0.cuda())
+        full.step(dt)
+        tfull, _ = time_steps(torch, lambda: full.step(dt), max(2, k // 4), 0)
+        del full, ref
+        torch.cuda.empty_cache()
+        v = cells * STAGES * k / tsel
+        v64 = cells * STAGES * k / t64
+        vl64 = cells * STAGES * k / tl64
+        res[f"fp32_learned_{M}"] = {
+            "value": v, "ms_per_step": 1e3 * tsel / k, "steps": k + 1, "grid": [M, M, M],
+            "flagged_frac_last_step": fr[-1], "skipped_frac_last_step": 1.0 - fr[-1],
+            "rel_l2_vs_fp64_weno": rel, "max_abs_vs_fp64_weno": mx,
+            "linear_only_rel_l2_vs_fp64_weno": rel_lin,
+            "fp64_weno_value": v64, "fp64_linear_value": vl64,
+            "fp32_all_weno_value": cells * STAGES * max(2, k // 4) / tfull,
+            "speedup_vs_fp64_weno": v / v64, "speedup_vs_fp64_linear": v / vl64}
+    return res
+
+
+def variant_ensemble(torch, acx, steps, nprob=48, seed=4848):
+    """48 independent ACOUSTIC4 problems, edges from {64, 96, 128, 160} per
+    axis, 1-3 Gaussian pulses each, FP32 compensated, persistent work-queue
+    kernel + captured two-step hipGraph, autotuned tile configuration; against
+    48 sequential single-problem FP32 steppers, and per-problem error against
+    the FP64 stepper."""
+    from acoustic_petsc_amd.ensemble import Ensemble, random_dims
+    dims = random_dims(nprob, seed)
+    h = 1.0 / 128
+    g0 = acx.Grid(n=(64, 64, 64), h=h, system=acx.ACOUSTIC4)
+    dt = acx.cfl_dt(g0, 0.4)
+    fields = [acx.field_gaussians(acx.Grid(n=d, h=h, system=acx.ACOUSTIC4), seed + p, npulse=1 + p % 3,
+                                  amp=(1e-4, 1e-3), sigma=(0.05, 0.15)) for p, d in enumerate(dims)]
+    e = Ensemble(dims, h)
+    e.load(fields)
+    best, ms_best = e.autotune(dt)
+    k = max(2, min(100, steps))
+    k += k % 2
+    t_ens, _ = time_steps(torch, lambda: e.graph_steps(dt, 2), k // 2, 1)
+    cells = e.ncells
+    v_ens = cells * STAGES * k / t_ens
+    # sequential single-problem launches
+    states = [acx.CompensatedState(acx.Grid(n=d, h=h, system=acx.ACOUSTIC4), torch.from_numpy(f).cuda())
+              for d, f in zip(dims, fields)]
+
+    def seq_step():
+        for s in states:
+            s.step(dt)
+    t_seq, _ = time_steps(torch, seq_step, k, 1)
+    v_seq = cells * STAGES * k / t_seq
+    # correctness: 10 steps from the initial fields, ensemble vs single-problem vs FP64
+    ne = 10
+    e.load(fields)
+    e.graph_steps(dt, ne)
+    hi, lo = e.state32()
+    hi, lo = hi.cpu().numpy(), lo.cpu().numpy()
+    bitwise, worst = True, 0.0
+    for p, (d, f) in enumerate(zip(dims, fields)):
+        g = acx.Grid(n=d, h=h, system=acx.ACOUSTIC4)
+        s = acx.CompensatedState(g, torch.from_numpy(f).cuda())
+        ref = torch.from_numpy(f).cuda()
+        w = (torch.empty_like(ref), torch.empty_like(ref))
+        for _ in range(ne):
+            s.step(dt)
+            acx.ssp_rks3_step(g, ref, dt, w)
+        a, b = 4 * e.offsets[p], 4 * e.offsets[p + 1]
+        bitwise &= bool((s.hi.cpu().numpy().ravel() == hi[a:b]).all() and (s.lo.cpu().numpy().ravel() == lo[a:b]).all())
+        rel, _ = acx.diff_f32c(torch.from_numpy(hi[a:b]).cuda(), torch.from_numpy(lo[a:b]).cuda(), ref.reshape(-1))
+        worst = max(worst, rel)
+    del states
+    e.close()
+    torch.cuda.empty_cache()
+    return {"value": v_ens, "ms_per_step": 1e3 * t_ens / k, "steps": k, "problems": nprob,
+            "cells": cells, "autotuned_config": best, "autotune_ms_per_step": ms_best,
+            "sequential_value": v_seq, "speedup_vs_sequential": v_ens / v_seq,
+            "hbm_frac": v_ens * f32c_bytes_per_update() / 1e9 / HBM_PEAK_GBS,
+            "check_steps": ne, "bitwise_equal_single_problem": bitwise,
+            "max_rel_l2_vs_fp64": worst}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -166,39 +345,22 @@ def main():
                      "bytes_per_cell_update": BYTES_FP64_AC4},
     }
 
-    if args.variants == "all":
+    if args.variants != "none":
+        want = set(VARIANTS) if args.variants == "all" else set(args.variants.split(","))
+        unknown = want - set(VARIANTS)
+        if unknown:
+            raise SystemExit(f"unknown variants {sorted(unknown)}; choose from {VARIANTS}")
         var = {}
-        # nonlinear CWENO4 FP64 on the same workload
-        gnl = acx.Grid(n=(N, N, N), h=1.0 / N, system=acx.ACOUSTIC4, gamma_hi=0.85, gamma_lo=0.85)
-        Unl = torch.from_numpy(U0).cuda()
-        k = max(2, min(10, args.steps))
-        t, _ = time_steps(torch, lambda: acx.ssp_rks3_step(gnl, Unl, dt, work), k, 1)
-        v = cells * STAGES * k / t
-        var["fp64_cweno_g085"] = {"value": v, "ms_per_step": 1e3 * t / k, "steps": k,
-                                  "hbm_frac": v * BYTES_FP64_AC4 / 1e9 / HBM_PEAK_GBS}
-        del Unl
-        # FP32 compensated at 384^3 (band-limited noise + pulse), error vs FP64
-        M = 384 if N == 256 else N
-        g3 = acx.Grid(n=(M, M, M), h=1.0 / M, system=acx.ACOUSTIC4)
-        V0 = acx.field_noise(g3, 3030, 4.0, 16.0, 1e-4, 1e-3, 0.05)
-        dt3 = acx.cfl_dt(g3, 0.4)
-        st = acx.CompensatedState(g3, torch.from_numpy(V0).cuda())
-        k3 = max(2, min(20, args.steps))
-        t, _ = time_steps(torch, lambda: st.step(dt3), k3, 1)
-        v = M ** 3 * STAGES * k3 / t
-        ref = torch.from_numpy(V0).cuda()
-        w3 = (torch.empty_like(ref), torch.empty_like(ref))
-        st_err = acx.CompensatedState(g3, ref)
-        for _ in range(k3):
-            acx.ssp_rks3_step(g3, ref, dt3, w3)
-            st_err.step(dt3)
-        rel, mx = acx.diff_f32c(st_err.hi, st_err.lo, ref)
-        var["fp32c_linear_384"] = {"value": v, "ms_per_step": 1e3 * t / k3, "steps": k3,
-                                   "bytes_per_cell_update": f32c_bytes_per_update(),
-                                   "hbm_frac": v * f32c_bytes_per_update() / 1e9 / HBM_PEAK_GBS,
-                                   "rel_l2_vs_fp64": rel, "max_abs_vs_fp64": mx,
-                                   "speedup_vs_fp64_linear": v / value}
-        del st, st_err, ref, w3
+        if "fp64_cweno" in want:
+            var["fp64_cweno_g085"] = variant_fp64_cweno(torch, acx, N, U0, work, dt, args.steps)
+        if "fp32c" in want:
+            var["fp32c_linear_384"] = variant_fp32c(torch, acx, 384 if N == 256 else N, args.steps, value)
+        del work, U
+        torch.cuda.empty_cache()
+        if "learned" in want:
+            var.update(variant_learned(torch, acx, args.learned_sizes, args.steps, var, value))
+        if "ensemble" in want:
+            var["ensemble_48"] = variant_ensemble(torch, acx, args.steps)
         out["variants"] = var
 
     if not args.no_cpu:

@@ -82,18 +82,22 @@ def test_backward_matches_cpu(acx, torch_gpu):
     err = np.linalg.norm(G - ref) / np.linalg.norm(ref)
     print(f"[case] backward rel err {err:.2e}", flush=True)
     assert err < 1e-5
-    # and a finite-difference check of the loss through the CPU model for one weight of each layer
+    # and a central-difference check of the FP64 loss (no bf16 rounding: the
+    # bf16 grid is coarser than the step) for weights of every layer; the
+    # gradient of the bf16 forward differs from it by the operand rounding
     def loss(pp):
-        yy, _, _ = cpu_forward(pp, X)
-        yy = np.clip(yy, 1e-12, 1 - 1e-12)
+        W1, b1, W2, b2, w3, b3 = split(pp)
+        x = X.astype(np.float64)
+        h2 = np.tanh(np.tanh(x @ W1.T + b1) @ W2.T + b2)
+        yy = np.clip(1.0 / (1.0 + np.exp(-(h2 @ w3 + b3))), 1e-12, 1 - 1e-12)
         return -np.sum(c * (t * np.log(yy) + (1 - t) * np.log(1 - yy))) / n
-    for k in (2144, 2120, 2090, 1100, 1060, 5):
-        e = 1e-3
+    for k in (2144, 2120, 2090, 1100, 1060, 1030, 5, 300):
+        e = 1e-4
         pp, pm = p.astype(np.float64).copy(), p.astype(np.float64).copy()
         pp[k] += e
         pm[k] -= e
         fd = (loss(pp) - loss(pm)) / (2 * e)
-        assert abs(fd - G[k]) < 0.05 * abs(fd) + 2e-4, (k, fd, G[k])
+        assert abs(fd - G[k]) < 0.03 * abs(fd) + 3e-4 * np.abs(G).max(), (k, fd, G[k])
 
 
 def test_labels_match_oracle_weights(acx, orc, torch_gpu):
@@ -135,7 +139,8 @@ def test_fused_indicator_equals_features_plus_forward(acx, torch_gpu):
 
 
 def test_selective_limits(acx, torch_gpu):
-    """All flags set == full CWENO4 FP32 step; no flag set == the linear scheme."""
+    """All flags set == full CWENO4 FP32 step, no flag set == the linear scheme
+    (both to FP32 rounding: the selective kernel is another instantiation)."""
     torch = torch_gpu
     N = 24
     gnl = acx.Grid(n=(N, N, N), h=1.0 / N, gamma_hi=0.85, gamma_lo=0.85)
@@ -154,7 +159,8 @@ def test_selective_limits(acx, torch_gpu):
         d.step(dt)   # linear kernel
     ra, rb = a.value64().cpu().numpy(), b.value64().cpu().numpy()
     rc, rd = c.value64().cpu().numpy(), d.value64().cpu().numpy()
-    assert np.array_equal(ra, rb)
+    # same arithmetic, different instantiations: equal to FP32 rounding
+    assert np.linalg.norm(ra - rb) / np.linalg.norm(rb) < 1e-6
     assert np.linalg.norm(rc - rd) / np.linalg.norm(rd) < 1e-6
     assert np.linalg.norm(ra - rc) / np.linalg.norm(rc) > 1e-6   # the fronts do trigger the weights
 
