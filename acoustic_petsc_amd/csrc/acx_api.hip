@@ -1,5 +1,6 @@
 // acx_api.hip -- the C ABI (include/acx.h): argument checks, kernel choice
 // and the SSP stage sequences.
+#include <mutex>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -207,6 +208,59 @@ int acx_ssp_rks3_step_f32c(const acx_grid *g, const float *HI, const float *LO, 
 
 } // extern "C"
 
+namespace acx {
+
+// Tile mask of the selective step: one byte per CWENO tile, kept per device
+// and grown on demand (selective steps issued concurrently on one device
+// share it and must not overlap).
+static uint8_t *sel_mask_scratch(size_t n)
+{
+    static std::mutex mu;
+    static uint8_t *buf[64] = {};
+    static size_t cap[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (cap[dev] < n) {
+        if (buf[dev]) (void)hipFree(buf[dev]);
+        buf[dev] = nullptr;
+        cap[dev] = 0;
+        if (hipMalloc(&buf[dev], n) != hipSuccess) return nullptr;
+        cap[dev] = n;
+    }
+    return buf[dev];
+}
+
+static void fill_cw_patch(CwArgs &a, const acx_grid *g)
+{
+    fill_cw(a, g);
+    const int zc = env_int("ACX_ZC_PATCH", 16);
+    a.zc = zc < 1 ? 1 : (zc > g->n[2] ? g->n[2] : zc);
+}
+
+// Selective stage: the linear kernel writes every cell, then the CWENO kernel
+// rewrites the tiles that hold a flagged cell or a face neighbour of one
+// (same stream, so the second pass lands last).
+static int stage32c_sel(const acx_grid *g, const float *HI, const float *D, const float *LO, float *OUT, float *OUT2,
+                        double b, double c, int final_stage, const uint8_t *flags, const uint8_t *mask, hipStream_t s)
+{
+    LinArgs la;
+    fill_lin(la, g);
+    la.q = HI; la.d = D; la.lo = LO; la.out = OUT; la.out2 = OUT2;
+    la.b = b; la.c = c; la.final_stage = final_stage;
+    int rc = launch_linear(la, g->system, EPI_F32C, true, s);
+    if (rc) return rc;
+    CwArgs ca;
+    fill_cw_patch(ca, g);
+    ca.q = HI; ca.d = D; ca.lo = LO; ca.out = OUT; ca.out2 = OUT2;
+    ca.b = b; ca.c = c; ca.final_stage = final_stage;
+    ca.flags = flags;
+    ca.tilemask = mask;
+    return launch_cweno(ca, g->system, EPI_F32C, true, true, s);
+}
+
+} // namespace acx
+
 // selective (learned-indicator) step, declared in acx_learn.h
 extern "C" int acx_ssp_rks3_step_f32c_sel(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
                                           float *LO_OUT, float *D0, float *D1, const uint8_t *flags,
@@ -217,9 +271,22 @@ extern "C" int acx_ssp_rks3_step_f32c_sel(const acx_grid *g, const float *HI, co
     if (!flags) return set_error(ACX_EINVAL, "flags is NULL");
     if (!HI || !LO || !HI_OUT || !LO_OUT || !D0 || !D1 || HI == HI_OUT || LO == LO_OUT)
         return set_error(ACX_EINVAL, "f32c step: outputs must not alias the inputs");
+    if (g->system != ACX_ACOUSTIC4) return set_error(ACX_EUNSUPPORTED, "selective step: ACOUSTIC4 only");
     hipStream_t s = (hipStream_t)stream;
-    if ((rc = stage32c(g, HI, nullptr, LO, D0, nullptr, 0.0, dt / 2, 0, flags, s))) return rc;
-    if ((rc = stage32c(g, HI, D0, LO, D1, nullptr, 1.0, dt / 2, 0, flags, s))) return rc;
-    if ((rc = stage32c(g, HI, D1, LO, D0, nullptr, 1.0 / 3.0, dt / 6, 0, flags, s))) return rc;
-    return stage32c(g, HI, D0, LO, HI_OUT, LO_OUT, 1.0, dt / 2, 1, flags, s);
+    CwArgs ca;
+    fill_cw_patch(ca, g);
+    if (env_int("ACX_SEL_FULL", 0)) {
+        // check mode: the CWENO kernel over every tile, no linear pass
+        if ((rc = stage32c(g, HI, nullptr, LO, D0, nullptr, 0.0, dt / 2, 0, flags, s))) return rc;
+        if ((rc = stage32c(g, HI, D0, LO, D1, nullptr, 1.0, dt / 2, 0, flags, s))) return rc;
+        if ((rc = stage32c(g, HI, D1, LO, D0, nullptr, 1.0 / 3.0, dt / 6, 0, flags, s))) return rc;
+        return stage32c(g, HI, D0, LO, HI_OUT, LO_OUT, 1.0, dt / 2, 1, flags, s);
+    }
+    uint8_t *mask = sel_mask_scratch(cweno_sel_tiles(ca));
+    if (!mask) return set_error(ACX_EHIP, "selective step: tile mask allocation failed");
+    if ((rc = launch_cweno_tilemask(ca, flags, mask, s))) return rc;
+    if ((rc = stage32c_sel(g, HI, nullptr, LO, D0, nullptr, 0.0, dt / 2, 0, flags, mask, s))) return rc;
+    if ((rc = stage32c_sel(g, HI, D0, LO, D1, nullptr, 1.0, dt / 2, 0, flags, mask, s))) return rc;
+    if ((rc = stage32c_sel(g, HI, D1, LO, D0, nullptr, 1.0 / 3.0, dt / 6, 0, flags, mask, s))) return rc;
+    return stage32c_sel(g, HI, D0, LO, HI_OUT, LO_OUT, 1.0, dt / 2, 1, flags, mask, s);
 }

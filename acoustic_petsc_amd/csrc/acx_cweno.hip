@@ -106,6 +106,9 @@ k_cweno(CwArgs p)
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
     const int tile = xcd_logical_tile(blockIdx.x, gridDim.x);
+    if constexpr (SEL) {
+        if (p.tilemask && !p.tilemask[tile]) return;   // whole block: the linear pass holds this tile
+    }
     const int gx0 = (tile % p.ntx) * (BX - 2) - 1, gy0 = ((tile / p.ntx) % p.nty) * (BY - 2) - 1;
     const int gx = gx0 + tx, gy = gy0 + ty;
     const int zb = (tile / (p.ntx * p.nty)) * p.zc, ze = min(zb + p.zc, nz);
@@ -357,10 +360,56 @@ k_cweno(CwArgs p)
 
 // ---------------------------------------------------------------------------
 
+constexpr int CW_BX = 32, CW_BY = 16;   // reconstruction columns per block (output tile 30 x 14)
+
+__global__ void k_fill_u8(uint8_t *m, size_t n, uint8_t v)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) m[i] = v;
+}
+
+// Marks the tile of every flagged cell and of its six face neighbours
+// (periodic wrap): their outputs use the flagged reconstruction.
+__global__ void k_tile_mark(const uint8_t *__restrict__ flags, int nx, int ny, int nz, int zc, int ntx, int nty,
+                            uint8_t *__restrict__ mask)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n || flags[c] == 0) return;
+    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
+    auto mark = [&](int x, int y, int z) {
+        x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+        y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+        z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+        const size_t t = ((size_t)(z / zc) * nty + y / (CW_BY - 2)) * ntx + x / (CW_BX - 2);
+        mask[t] = 1;
+    };
+    mark(i, j, k);
+    mark(i - 1, j, k); mark(i + 1, j, k);
+    mark(i, j - 1, k); mark(i, j + 1, k);
+    mark(i, j, k - 1); mark(i, j, k + 1);
+}
+
+size_t cweno_sel_tiles(const CwArgs &a)
+{
+    const size_t ntx = (a.nx + CW_BX - 3) / (CW_BX - 2), nty = (a.ny + CW_BY - 3) / (CW_BY - 2);
+    return ntx * nty * ((a.nz + a.zc - 1) / a.zc);
+}
+
+int launch_cweno_tilemask(const CwArgs &a, const uint8_t *flags, uint8_t *mask, hipStream_t s)
+{
+    const int ntx = (a.nx + CW_BX - 3) / (CW_BX - 2), nty = (a.ny + CW_BY - 3) / (CW_BY - 2);
+    const size_t nt = cweno_sel_tiles(a), n = (size_t)a.nx * a.ny * a.nz;
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, mask, nt, (uint8_t)0);
+    hipLaunchKernelGGL(k_tile_mark, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, flags, a.nx, a.ny, a.nz, a.zc,
+                       ntx, nty, mask);
+    return hip_check(hipGetLastError(), "tile mask launch");
+}
+
 template <typename T, int SYS, int EPI, bool SEL>
 static int launch_cw_t(const CwArgs &a, hipStream_t s)
 {
-    constexpr int BX = 32, BY = 16;
+    constexpr int BX = CW_BX, BY = CW_BY;
     CwArgs b = a;
     b.ntx = (a.nx + BX - 3) / (BX - 2);
     b.nty = (a.ny + BY - 3) / (BY - 2);

@@ -40,9 +40,10 @@ __device__ __forceinline__ void axis_linear(T m2, T m1, T u0, T p1, T p2, T &c1,
     c3 = T(1.0 / 12.0) * (-m2 + T(2) * (m1 - p1) + p2);
 }
 
-// Full CWENO4: returns the axis coefficients (c1,c2,c3 per axis) and wt_ratio.
+// Full CWENO4: returns the axis coefficients (c1,c2,c3 per axis) and wt_ratio;
+// wn (if not null) receives the eight normalised weights (R4, R3[0..6]).
 template <typename T>
-__device__ __forceinline__ T cweno_axes(const T (&s)[33], const CwArgs &p, Axis3<T> &o)
+__device__ __forceinline__ T cweno_axes(const T (&s)[33], const CwArgs &p, Axis3<T> &o, T *wn = nullptr)
 {
     // stencil slots (see gather order in the kernel)
     const T u0 = s[0];
@@ -139,6 +140,12 @@ __device__ __forceinline__ T cweno_axes(const T (&s)[33], const CwArgs &p, Axis3
         wt = w4 * it * T(p.inv_g4);
 #pragma unroll
         for (int k = 0; k < 7; ++k) w3[k] *= it;
+    }
+
+    if (wn) {
+        wn[0] = wt * T(p.g4);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) wn[1 + k] = w3[k];
     }
 
     // CWENO combination, reconstruction.c:223-257, for the face-relevant

@@ -33,6 +33,7 @@ struct CwArgs {
     void *out;
     void *out2;
     const uint8_t *flags;   // selective mode: per-cell bitmask, bit v set => full CWENO for component v
+    const uint8_t *tilemask;   // selective mode: tiles with a zero entry are left to the linear pass
     int nx, ny, nz, zc;
     int ntx, nty, ntz;
     int bc[3];
@@ -44,6 +45,10 @@ struct CwArgs {
 
 int launch_linear(const LinArgs &a, int sys, int epi, bool fp32, hipStream_t s);
 int launch_cweno(const CwArgs &a, int sys, int epi, bool fp32, bool selective, hipStream_t s);
+// Tiles of the FP32 selective CWENO launch for a.zc: count, and a mask that is
+// 1 for every tile holding a flagged cell or a face neighbour of one.
+size_t cweno_sel_tiles(const CwArgs &a);
+int launch_cweno_tilemask(const CwArgs &a, const uint8_t *flags, uint8_t *mask, hipStream_t s);
 
 int set_error(int code, const char *fmt, ...);
 // integer tuning knob from the environment (dflt when unset or empty)

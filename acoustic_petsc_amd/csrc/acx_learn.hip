@@ -32,6 +32,15 @@ __host__ __device__ inline int kperm(int g, int j) { return j < 4 ? 4 * g + j : 
 
 __device__ __forceinline__ float bf16_round(float x) { return (float)(__bf16)x; }
 
+// tanh(x) = sign(x) (1 - e) / (1 + e), e = exp(-2|x|) <= 1: one exponential
+// and one reciprocal (absolute error ~1e-7, far below the bf16 rounding the
+// layers apply to their operands).
+__device__ __forceinline__ float act_tanh(float x)
+{
+    const float e = __expf(-2.f * fabsf(x));
+    return copysignf((1.f - e) * __builtin_amdgcn_rcpf(1.f + e), x);
+}
+
 // ---- features --------------------------------------------------------------
 
 // a[axis][k]: value at offset k-2 along the axis.  18 normalised undivided
@@ -112,9 +121,13 @@ __global__ void __launch_bounds__(256) k_labels(const double *__restrict__ U, Cw
         for (int q = 0; q < 33; ++q)
             st[q] = U[(((size_t)W(k + dz[q], nz) * ny + W(j + dy[q], ny)) * nx + W(i + dx[q], nx)) * 4 + v];
         Axis3<double> ax;
-        const double w = cweno_axes<double>(st, p, ax);
-        labels[cell * 4 + v] = fabs(w - 1.0) > thr ? 1 : 0;
-        if (wt) wt[cell * 4 + v] = (float)w;
+        double wn[8];
+        cweno_axes<double>(st, p, ax, wn);
+        double dev = fabs(wn[0] - p.g4) + fabs(wn[1] - p.g3c);
+#pragma unroll
+        for (int q = 2; q < 8; ++q) dev += fabs(wn[q] - p.g3s);
+        labels[cell * 4 + v] = dev > thr ? 1 : 0;
+        if (wt) wt[cell * 4 + v] = (float)dev;
     }
 }
 
@@ -130,8 +143,8 @@ __device__ __forceinline__ float mlp_wave(const MlpPacked &P, const bf16x8 &x, i
     float h[8];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        h[i] = tanhf(a0[i] + P.b1[4 * g + i]);
-        h[4 + i] = tanhf(a1[i] + P.b1[16 + 4 * g + i]);
+        h[i] = act_tanh(a0[i] + P.b1[4 * g + i]);
+        h[4 + i] = act_tanh(a1[i] + P.b1[16 + 4 * g + i]);
     }
     bf16x8 hb;
 #pragma unroll
@@ -145,8 +158,8 @@ __device__ __forceinline__ float mlp_wave(const MlpPacked &P, const bf16x8 &x, i
     float acc = 0.f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const float u0 = tanhf(c0[i] + P.b2[4 * g + i]);
-        const float u1 = tanhf(c1[i] + P.b2[16 + 4 * g + i]);
+        const float u0 = act_tanh(c0[i] + P.b2[4 * g + i]);
+        const float u1 = act_tanh(c1[i] + P.b2[16 + 4 * g + i]);
         if (h2out) { (*h2out)[i] = u0; (*h2out)[4 + i] = u1; }
         acc += P.w3[4 * g + i] * u0 + P.w3[16 + 4 * g + i] * u1;
     }
@@ -185,42 +198,58 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const MlpPacked *__restrict
 
 // ---- fused grid inference -------------------------------------------------------
 
-__global__ void __launch_bounds__(256) k_indicator(const MlpPacked *__restrict__ Pg, const float *__restrict__ HI,
-                                                   const float *__restrict__ D, int nx, int ny, int nz, float thr,
-                                                   uint8_t *__restrict__ flags)
+constexpr int IND_WAVES = 4;
+
+// Each wave pass covers 16 cells x 4 components = 64 samples: lane l forms
+// the features of sample l (cell c0 + l/4, component l%4; the 64 lanes read
+// 64 consecutive floats per stencil point), parks them in LDS, and the wave
+// then runs four 16-sample MFMA batches over them.
+__global__ void __launch_bounds__(64 * IND_WAVES) k_indicator(const MlpPacked *__restrict__ Pg,
+                                                             const float *__restrict__ HI,
+                                                             const float *__restrict__ D, int nx, int ny, int nz,
+                                                             float thr, uint8_t *__restrict__ flags)
 {
     __shared__ MlpPacked P;
+    __shared__ float feat[IND_WAVES][64][ACX_MLP_IN + 1];
     for (int q = threadIdx.x; q < (int)(sizeof(MlpPacked) / 4); q += blockDim.x)
         reinterpret_cast<float *>(&P)[q] = reinterpret_cast<const float *>(Pg)[q];
     __syncthreads();
     const size_t ncell = (size_t)nx * ny * nz;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4, sl = lane & 15;
-    const size_t nw = (size_t)gridDim.x * (blockDim.x >> 6);
-    // each wave pass: 4 cells x 4 components = 16 samples
-    for (size_t c0 = ((size_t)blockIdx.x * (blockDim.x >> 6) + wv) * 4; c0 < ncell; c0 += nw * 4) {
-        const size_t cell = c0 + (sl >> 2);
-        const int v = sl & 3;
-        bf16x8 x;
-        if (cell < ncell) {
-            const int i = (int)(cell % nx), j = (int)((cell / nx) % ny), k = (int)(cell / ((size_t)nx * ny));
-            float a[3][5], f[ACX_MLP_NFEAT];
-            gather_arms(HI, D, nx, ny, nz, i, j, k, v, a);
-            features19(a, f);
+    float(*F)[ACX_MLP_IN + 1] = feat[wv];
+    const size_t nw = (size_t)gridDim.x * IND_WAVES;
+    for (size_t c0 = ((size_t)blockIdx.x * IND_WAVES + wv) * 16; c0 < ncell; c0 += nw * 16) {
+        {
+            const size_t cell = c0 + (lane >> 2);
+            const int v = lane & 3;
+            float f[ACX_MLP_NFEAT];
+            if (cell < ncell) {
+                const int i = (int)(cell % nx), j = (int)((cell / nx) % ny), k = (int)(cell / ((size_t)nx * ny));
+                float a[3][5];
+                gather_arms(HI, D, nx, ny, nz, i, j, k, v, a);
+                features19(a, f);
+            } else {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int fi = 8 * g + q;
-                float val = 0.f;
-#pragma unroll
-                for (int r = 0; r < ACX_MLP_NFEAT; ++r) val = (r == fi) ? f[r] : val;
-                x[q] = (__bf16)val;
+                for (int q = 0; q < ACX_MLP_NFEAT; ++q) f[q] = 0.f;
             }
-        } else {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) x[q] = (__bf16)0.f;
+            for (int q = 0; q < ACX_MLP_IN; ++q) F[lane][q] = q < ACX_MLP_NFEAT ? f[q] : 0.f;
         }
-        const float y = mlp_wave(P, x, lane, nullptr, nullptr);
-        const unsigned long long mask = __ballot(y > thr);
-        if (lane < 4 && c0 + lane < ncell) flags[c0 + lane] = (uint8_t)((mask >> (4 * lane)) & 0xF);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            bf16x8 x;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) x[q] = (__bf16)F[16 * b + sl][8 * g + q];
+            const float y = mlp_wave(P, x, lane, nullptr, nullptr);
+            const unsigned long long m = __ballot(y > thr);   // bit n (< 16): sample 16b + n
+            const size_t cell = c0 + 4 * b + lane;
+            if (lane < 4 && cell < ncell) flags[cell] = (uint8_t)((m >> (4 * lane)) & 0xF);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();   // every read of F is done before the next pass writes it
     }
 }
 
@@ -458,10 +487,11 @@ int acx_indicator_flags(const acx_grid *g, const float *HI, const float *D, cons
     if (rc) return rc;
     if (!HI || !packed || !flags) return set_error(ACX_EINVAL, "indicator: null pointer");
     const size_t nc = (size_t)g->n[0] * g->n[1] * g->n[2];
-    const size_t groups = (nc + 3) / 4;   // one wave pass per 4 cells
-    size_t blocks = (groups + 3) / 4;
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_indicator, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, (const MlpPacked *)packed,
+    const size_t groups = (nc + 15) / 16;   // one wave pass per 16 cells
+    size_t blocks = (groups + IND_WAVES - 1) / IND_WAVES;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_indicator, dim3((unsigned)blocks), dim3(64 * IND_WAVES), 0, (hipStream_t)stream,
+                       (const MlpPacked *)packed,
                        HI, D, g->n[0], g->n[1], g->n[2], thr, flags);
     return hip_check(hipGetLastError(), "indicator");
 }

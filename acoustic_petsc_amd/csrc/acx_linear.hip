@@ -20,6 +20,7 @@
 
 namespace acx {
 
+// PF = 0: centre-plane march; PF >= 1: lead-plane march with loads PF planes ahead.
 template <typename T, int SYS, int BX, int BY, int EPI, int PF, int WPE>
 __global__ void __launch_bounds__(BX *BY) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_linear(LinArgs p)
@@ -27,16 +28,18 @@ k_linear(LinArgs p)
     using Tile = LinTile<T, SYS, BX, BY>;
     __shared__ T lds[2][Tile::NS][BY + 6][BX + 6];
     const int tile = xcd_logical_tile(blockIdx.x, gridDim.x);
-    linear_tile<T, SYS, BX, BY, EPI, PF>(p, (tile % p.ntx) * BX, ((tile / p.ntx) % p.nty) * BY,
-                                         (tile / (p.ntx * p.nty)) * p.zc, lds);
+    const int x0 = (tile % p.ntx) * BX, y0 = ((tile / p.ntx) % p.nty) * BY, zb = (tile / (p.ntx * p.nty)) * p.zc;
+    if constexpr (PF == 0)
+        linear_tile_centre<T, SYS, BX, BY, EPI>(p, x0, y0, zb, lds);
+    else
+        linear_tile<T, SYS, BX, BY, EPI, PF>(p, x0, y0, zb, lds);
 }
 
 // ---------------------------------------------------------------------------
 
-template <typename T, int SYS, int EPI, int PF, int WPE>
+template <typename T, int SYS, int EPI, int PF, int WPE, int BX, int BY>
 static int launch_linear_v(const LinArgs &a, hipStream_t s)
 {
-    constexpr int BX = 64, BY = 8;
     LinArgs b = a;
     b.ntx = (a.nx + BX - 1) / BX;
     b.nty = (a.ny + BY - 1) / BY;
@@ -47,18 +50,26 @@ static int launch_linear_v(const LinArgs &a, hipStream_t s)
     return hip_check(hipGetLastError(), "k_linear launch");
 }
 
-// Prefetch depth / occupancy variants (ACX_LIN_VARIANT): 0 = one plane ahead,
-// 2 waves per SIMD; 1 = one plane ahead, registers capped for 4 waves per SIMD;
-// 2 = two planes ahead; 3 = three planes ahead.
+// Launch variants (ACX_LIN_VARIANT): lead-plane march, tile 64 x 8, loads 1
+// plane ahead (0: 2 waves per SIMD, 1: registers capped for 4 waves per
+// SIMD), 2 planes ahead (2) or 3 planes ahead (3); tile 32 x 16 with loads 2
+// (4), 1 (5) or 3 (6) planes ahead; centre-plane march, tile 64 x 8 (7) or
+// 32 x 16 (8), centre-plane 64 x 8 capped for 4 waves per SIMD (9).
 template <typename T, int SYS, int EPI>
 static int launch_linear_t(const LinArgs &a, hipStream_t s)
 {
-    static const int var = env_int("ACX_LIN_VARIANT", sizeof(T) == 8 ? 2 : 1);
+    static const int var = env_int("ACX_LIN_VARIANT", sizeof(T) == 8 ? 7 : 0);
     switch (var) {
-    case 0: return launch_linear_v<T, SYS, EPI, 1, 1>(a, s);
-    case 1: return launch_linear_v<T, SYS, EPI, 1, 4>(a, s);
-    case 3: return launch_linear_v<T, SYS, EPI, 3, 1>(a, s);
-    default: return launch_linear_v<T, SYS, EPI, 2, 1>(a, s);
+    case 0: return launch_linear_v<T, SYS, EPI, 1, 1, 64, 8>(a, s);
+    case 1: return launch_linear_v<T, SYS, EPI, 1, 4, 64, 8>(a, s);
+    case 3: return launch_linear_v<T, SYS, EPI, 3, 1, 64, 8>(a, s);
+    case 4: return launch_linear_v<T, SYS, EPI, 2, 1, 32, 16>(a, s);
+    case 5: return launch_linear_v<T, SYS, EPI, 1, 1, 32, 16>(a, s);
+    case 6: return launch_linear_v<T, SYS, EPI, 3, 1, 32, 16>(a, s);
+    case 7: return launch_linear_v<T, SYS, EPI, 0, 1, 64, 8>(a, s);
+    case 8: return launch_linear_v<T, SYS, EPI, 0, 1, 32, 16>(a, s);
+    case 9: return launch_linear_v<T, SYS, EPI, 0, 4, 64, 8>(a, s);
+    default: return launch_linear_v<T, SYS, EPI, 2, 1, 64, 8>(a, s);
     }
 }
 

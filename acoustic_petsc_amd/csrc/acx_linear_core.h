@@ -279,4 +279,207 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
     }
 }
 
+// Centre-plane march of the same tile: the own column's z-queue holds planes
+// z-3 .. z+3 before plane z is computed, and plane z (with its x/y halo) is
+// staged in LDS and finished in the same iteration.  Fewer live registers
+// than the lead-plane march (no queue of in-plane parts), at the price of
+// halo loads one plane ahead of the neighbours' interior loads.  The column
+// (4 planes ahead) and halo (1 plane ahead) loads keep their raw words until
+// the end of the iteration, so their latency overlaps the iteration's work.
+template <typename T, int SYS, int BX, int BY, int EPI>
+__device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int y0, int zb,
+                                                   T (&lds)[2][LinTile<T, SYS, BX, BY>::NS][BY + 6][BX + 6])
+{
+    using Tile = LinTile<T, SYS, BX, BY>;
+    constexpr int NV = Tile::NV, NS = Tile::NS;
+    static_assert(BX * BY >= Tile::NHALO, "every halo cell of a plane needs a thread of its own");
+
+    const T *__restrict__ Q = static_cast<const T *>(p.q);
+    const T *__restrict__ D = static_cast<const T *>(p.d);
+    const T *__restrict__ U0 = static_cast<const T *>(p.u0);
+    const T *__restrict__ LO = static_cast<const T *>(p.lo);
+    T *__restrict__ OUT = static_cast<T *>(p.out);
+    T *__restrict__ OUT2 = static_cast<T *>(p.out2);
+
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+    const int ze = min(zb + p.zc, nz);
+    const bool active = (x0 + tx < nx) && (y0 + ty < ny);
+    const size_t plane = (size_t)nx * ny;
+    const bool has_d = (EPI == EPI_F32C) && D != nullptr;
+
+    float sgx = 1.f, sgy = 1.f;
+    const int mx = map_axis(x0 + tx, nx, p.bc[0] == ACX_PERIODIC, sgx);
+    const int my = map_axis(y0 + ty, ny, p.bc[1] == ACX_PERIODIC, sgy);
+    const size_t col = (size_t)my * nx + mx;
+    const float csign = sgx * sgy;
+
+    int hx = -1, hy = -1;
+    size_t hcol = 0;
+    float hsign = 1.f;
+    if (tid < Tile::NHALO) {
+        int t = tid;
+        if (t < 6 * BY) {
+            int row = t / 6, k = t % 6;
+            hy = row + 3;
+            hx = k < 3 ? k : BX + k;
+        } else {
+            t -= 6 * BY;
+            int colx = t % BX, k = t / BX;
+            hx = colx + 3;
+            hy = k < 3 ? k : BY + k;
+        }
+        float s1 = 1.f, s2 = 1.f;
+        int gx = map_axis(x0 + hx - 3, nx, p.bc[0] == ACX_PERIODIC, s1);
+        int gy = map_axis(y0 + hy - 3, ny, p.bc[1] == ACX_PERIODIC, s2);
+        hcol = (size_t)gy * nx + gx;
+        hsign = s1 * s2;
+    }
+    auto zmap = [&](int z, float &s) -> size_t {
+        s = 1.f;
+        return (size_t)map_axis(z, nz, p.bc[2] == ACX_PERIODIC, s) * plane;
+    };
+    struct Raw {
+        Cell<T, NV> q, d;
+        float s;
+    };
+    auto issue = [&](Raw &r, size_t c, int z) {
+        const size_t off = zmap(z, r.s);
+        r.q = load_cell<T, NV>(Q, off + c);
+        if (has_d) r.d = load_cell<T, NV>(D, off + c);
+    };
+    auto apply = [&](const Raw &r, float sign) -> Cell<T, NV> {
+        Cell<T, NV> c = r.q;
+        if (has_d) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) c.v[v] += r.d.v[v];
+        }
+        c.v[0] *= T(r.s * sign);
+        return c;
+    };
+
+    Cell<T, NV> qz[7];
+#pragma unroll
+    for (int d = -3; d <= 3; ++d) {
+        Raw r;
+        issue(r, col, zb + d);
+        qz[d + 3] = apply(r, csign);
+    }
+    Cell<T, NV> hv;
+    if (hx >= 0) {
+        Raw r;
+        issue(r, hcol, zb);
+        hv = apply(r, hsign);
+    }
+
+    const T inv48h = T(p.inv48h), Kc = T(p.K), irho = T(p.invrho), sLLF = T(p.s);
+    const T lap = T(p.lapc);
+
+    int buf = 0;
+    for (int z = zb; z < ze; ++z) {
+        Raw qn, hn;
+        issue(qn, col, z + 4);
+        if (hx >= 0) issue(hn, hcol, z + 1);
+        const size_t idx = (size_t)z * plane + col;
+        Cell<T, NV> e0, e1, e2;
+        if (active) {
+            if constexpr (EPI == EPI_COMB) e0 = load_cell<T, NV>(U0, idx);
+            if constexpr (EPI == EPI_F32C) {
+                if (has_d) e0 = load_cell<T, NV>(D, idx);
+                if (p.final_stage) {
+                    e1 = load_cell<T, NV>(LO, idx);
+                    e2 = load_cell<T, NV>(Q, idx);
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NS; ++v) lds[buf][v][ty + 3][tx + 3] = qz[3].v[v];
+        if (hx >= 0) {
+#pragma unroll
+            for (int v = 0; v < NS; ++v) lds[buf][v][hy][hx] = hv.v[v];
+        }
+        __syncthreads();
+
+        Cell<T, NV> r;
+        if constexpr (SYS == ACX_ACOUSTIC4) {
+            T accB[4], dAp[3], dAvel = T(0);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const T *row = &lds[buf][v][ty + 3][tx + 3];
+                const T xm3 = row[-3], xm2 = row[-2], xm1 = row[-1], xp1 = row[1], xp2 = row[2], xp3 = row[3];
+                const T ym3 = lds[buf][v][ty][tx + 3], ym2 = lds[buf][v][ty + 1][tx + 3];
+                const T ym1 = lds[buf][v][ty + 2][tx + 3];
+                const T yp1 = lds[buf][v][ty + 4][tx + 3], yp2 = lds[buf][v][ty + 5][tx + 3];
+                const T yp3 = lds[buf][v][ty + 6][tx + 3];
+                const T q0 = qz[3].v[v];
+                accB[v] = stencil_B7(xm3, xm2, xm1, q0, xp1, xp2, xp3) + stencil_B7(ym3, ym2, ym1, q0, yp1, yp2, yp3) +
+                          stencil_B7(qz[0].v[v], qz[1].v[v], qz[2].v[v], q0, qz[4].v[v], qz[5].v[v], qz[6].v[v]);
+                if (v == 0) {
+                    dAp[0] = stencil_A7(xm3, xm2, xm1, xp1, xp2, xp3);
+                    dAp[1] = stencil_A7(ym3, ym2, ym1, yp1, yp2, yp3);
+                    dAp[2] = stencil_A7(qz[0].v[0], qz[1].v[0], qz[2].v[0], qz[4].v[0], qz[5].v[0], qz[6].v[0]);
+                } else if (v == 1) {
+                    dAvel += stencil_A7(xm3, xm2, xm1, xp1, xp2, xp3);
+                } else if (v == 2) {
+                    dAvel += stencil_A7(ym3, ym2, ym1, yp1, yp2, yp3);
+                } else {
+                    dAvel += stencil_A7(qz[0].v[3], qz[1].v[3], qz[2].v[3], qz[4].v[3], qz[5].v[3], qz[6].v[3]);
+                }
+            }
+            r.v[0] = inv48h * (sLLF * accB[0] - Kc * dAvel);
+            r.v[1] = inv48h * (sLLF * accB[1] - irho * dAp[0]);
+            r.v[2] = inv48h * (sLLF * accB[2] - irho * dAp[1]);
+            r.v[3] = inv48h * (sLLF * accB[3] - irho * dAp[2]);
+        } else {
+            const T *row = &lds[buf][0][ty + 3][tx + 3];
+            const T q0 = qz[3].v[0];
+            const T L = stencil_L7(row[-3], row[-2], row[-1], q0, row[1], row[2], row[3]) +
+                        stencil_L7(lds[buf][0][ty][tx + 3], lds[buf][0][ty + 1][tx + 3], lds[buf][0][ty + 2][tx + 3], q0,
+                                   lds[buf][0][ty + 4][tx + 3], lds[buf][0][ty + 5][tx + 3], lds[buf][0][ty + 6][tx + 3]) +
+                        stencil_L7(qz[0].v[0], qz[1].v[0], qz[2].v[0], q0, qz[4].v[0], qz[5].v[0], qz[6].v[0]);
+            r.v[0] = qz[3].v[1];     // du/dt = v (rhs_function.c:330)
+            r.v[1] = lap * L;
+        }
+
+        if (active) {
+            Cell<T, NV> o;
+            if constexpr (EPI == EPI_RHS) {
+                o = r;
+            } else if constexpr (EPI == EPI_AXPY) {
+                const T cc = T(p.c);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) o.v[v] = qz[3].v[v] + cc * r.v[v];
+            } else if constexpr (EPI == EPI_COMB) {
+                const T aa = T(p.a), bb = T(p.b), cc = T(p.c);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) o.v[v] = aa * e0.v[v] + bb * qz[3].v[v] + cc * r.v[v];
+            } else {
+                const T bb = T(p.b), cc = T(p.c);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) o.v[v] = (has_d ? bb * e0.v[v] : T(0)) + cc * r.v[v];
+                if (p.final_stage) {
+                    Cell<T, NV> nh, nl;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        const float t = (float)e1.v[v] + (float)o.v[v];
+                        float sm, er;
+                        two_sum((float)e2.v[v], t, sm, er);
+                        nh.v[v] = sm;
+                        nl.v[v] = er;
+                    }
+                    store_cell<T, NV>(OUT2, idx, nl);
+                    o = nh;
+                }
+            }
+            store_cell<T, NV>(OUT, idx, o);
+        }
+
+#pragma unroll
+        for (int k = 0; k < 6; ++k) qz[k] = qz[k + 1];
+        qz[6] = apply(qn, csign);
+        if (hx >= 0) hv = apply(hn, hsign);
+        buf ^= 1;
+    }
+}
+
 } // namespace acx

@@ -4,9 +4,11 @@ selective FP32 stepper that uses it (include/acx_learn.h).
 
 No counterpart in the reference (it always evaluates the full CWENO4 weights,
 3d/weno4/plane_wave/reconstruction.c:38-259).  The label of a (cell,
-component) is |wt_ratio - 1| > 0.05 with the nonlinear weights
-gamma = (0.85, 0.85) (Viscous-Multiphase-Fourth-Order/reconstruction.c:42-43),
-wt_ratio as in reconstruction.c:236.
+component) is sum_k |w_k - gamma_k| > 0.05 over the eight normalised CWENO
+weights (R4 and seven R3 stencils, reconstruction.c:205-218) with the
+nonlinear linear-weight setting gamma = (0.85, 0.85)
+(Viscous-Multiphase-Fourth-Order/reconstruction.c:42-43).  With w = gamma the
+CWENO combination (reconstruction.c:223-257) is exactly the R4 polynomial.
 """
 import ctypes as _C
 import math
@@ -82,7 +84,7 @@ def features(grid, U32, stream=None):
 
 
 def labels(grid, U64, thr=0.05, stream=None):
-    """(labels uint8 [ncells*4], wt_ratio float32 [ncells*4]) with the grid's gammas."""
+    """(labels uint8 [ncells*4], weight deviation float32 [ncells*4]) with the grid's gammas."""
     import torch
     L = _bind()
     T = torch.empty(grid.ncells * 4, dtype=torch.uint8, device="cuda")

@@ -4,10 +4,12 @@
  * No counterpart in the reference: acoustic_petsc always runs the full
  * CWENO4 weights (3d/weno4/plane_wave/reconstruction.c:38-259 and its copies).  This adds a small
  * MLP that predicts, per cell and component, whether the CWENO weights leave
- * the linear weights (|wt_ratio - 1| > threshold, wt_ratio as in
- * reconstruction.c:236), so smooth cells can take the linear 4th-order
- * polynomial while full CWENO stays the fallback everywhere the network is
- * not confident.
+ * the linear weights: the L1 distance sum_k |w_k - gamma_k| over the eight
+ * normalised weights (R4 and the seven R3 stencils,
+ * reconstruction.c:205-218) exceeds a threshold.  With w = gamma the CWENO
+ * combination (reconstruction.c:223-257) is exactly the R4 polynomial, and its
+ * change is bounded by that distance, so cells below it can take the linear
+ * 4th-order polynomial while full CWENO stays the fallback everywhere else.
  *
  * Network: 19 inputs (18 normalised undivided differences, 6 per axis, plus
  * a log-amplitude term), padded to 32 -> 32 tanh -> 32 tanh -> 1 sigmoid.
@@ -41,9 +43,9 @@ int acx_mlp_pack(const float *params, void *packed, void *stream);
 int acx_features_f32(const acx_grid *g, const float *U, float *X, void *stream);
 
 /* Training labels from the solver's own weights: for every (cell, component)
- * wt_ratio of the CWENO4 reconstruction with the grid's gamma_hi/gamma_lo
- * (FP64 state U), label = |wt_ratio - 1| > thr.  wt may be NULL. */
-int acx_weno_labels_f64(const acx_grid *g, const double *U, double thr, uint8_t *labels, float *wt, void *stream);
+ * of the FP64 state U, dev = sum_k |w_k - gamma_k| of the CWENO4 weights with
+ * the grid's gamma_hi/gamma_lo, label = dev > thr.  dev may be NULL. */
+int acx_weno_labels_f64(const acx_grid *g, const double *U, double thr, uint8_t *labels, float *dev, void *stream);
 
 /* MFMA forward of n samples X[n][32]: Y[n] = sigmoid output; H1, H2 (n x 32,
  * may be NULL) keep the activations for the backward pass. */
@@ -74,7 +76,11 @@ int acx_flags_count(const uint8_t *flags, size_t ncells, long long *bits, void *
 
 /* One compensated FP32 SSP-RK3 step with the selective reconstruction:
  * full CWENO4 (grid gammas) where the flag bit of a component is set, the
- * linear 4th-order polynomial elsewhere. */
+ * linear 4th-order polynomial elsewhere.  Each stage runs the linear-scheme
+ * kernel over every cell, then the CWENO kernel over the tiles (30 x 14 x 16
+ * cells) that hold a flagged cell or a face neighbour of one.  The tile mask
+ * is library scratch kept per device: selective steps on one device must not
+ * run concurrently.  ACOUSTIC4, periodic. */
 int acx_ssp_rks3_step_f32c_sel(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
                                float *LO_OUT, float *D0, float *D1, const uint8_t *flags, double dt, void *stream);
 

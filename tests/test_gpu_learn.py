@@ -101,6 +101,8 @@ def test_backward_matches_cpu(acx, torch_gpu):
 
 
 def test_labels_match_oracle_weights(acx, orc, torch_gpu):
+    """Label = L1 distance of the eight normalised CWENO weights from the
+    linear weights, against the oracle's weights of the same 5^3 blocks."""
     torch = torch_gpu
     from acoustic_petsc_amd import learn
     N = 12
@@ -111,12 +113,15 @@ def test_labels_match_oracle_weights(acx, orc, torch_gpu):
     T, W = learn.labels(grid, torch.from_numpy(U).cuda(), thr=0.05)
     W = W.cpu().numpy().reshape(N, N, N, 4)
     T = T.cpu().numpy().reshape(N, N, N, 4)
+    g4 = 0.85
+    gam = np.array([g4, (1 - g4) * 0.85] + [(1 - g4) * (1 - 0.85) / 6] * 6)
     for (k, j, i, v) in [(0, 0, 0, 0), (3, 5, 5, 1), (6, 6, 6, 2), (11, 2, 9, 3), (4, 7, 1, 0)]:
         blk = np.take(np.take(np.take(U[..., v], range(k - 2, k + 3), axis=0, mode="wrap"),
                               range(j - 2, j + 3), axis=1, mode="wrap"), range(i - 2, i + 3), axis=2, mode="wrap")
         _, dg = orc.cweno3d(blk, (0.85, 0.85))
-        assert abs(W[k, j, i, v] - dg[0]) < 1e-6 * abs(dg[0])
-        assert T[k, j, i, v] == (abs(dg[0] - 1) > 0.05)
+        dev = np.abs(np.array(dg[1:9]) - gam).sum()
+        assert abs(W[k, j, i, v] - dev) < 1e-6 * max(dev, 1e-3)
+        assert T[k, j, i, v] == (dev > 0.05)
     assert 0 < T.mean() < 1
 
 
@@ -166,19 +171,21 @@ def test_selective_limits(acx, torch_gpu):
 
 
 def test_trained_indicator_gain(acx, torch_gpu):
-    """Train on solver-labelled 32^3 snapshots (other seeds), then run a held-out
-    steep pulse: the learned stepper must skip most of the WENO work and stay
-    much closer to all-WENO than the purely linear scheme does."""
+    """Train on solver-labelled 64^3 snapshots (other seeds), then run a
+    held-out steep pulse: the learned stepper must skip most of the WENO work
+    (the method without the indicator evaluates it everywhere) and stay far
+    closer to all-WENO FP64 than the linear scheme does."""
     torch = torch_gpu
     from acoustic_petsc_amd import learn
-    X, T = learn.make_dataset(n_snapshots=6, N=32, seed0=300)
+    X, T = learn.make_dataset(n_snapshots=4, N=64, seed0=300)
     print(f"[case] dataset {X.shape[0]} samples, positive {float(T.float().mean()):.3f}", flush=True)
-    p = learn.train(X, T, epochs=6, batch=4096, lr=3e-3, seed=1)
+    p = learn.train(X, T, epochs=4, batch=8192, lr=3e-3, seed=1)
     packed = learn.pack(p)
     rec, prec, frac = learn.evaluate(packed, X, T, 0.3)
     print(f"[case] train recall {rec:.3f} precision {prec:.3f} flagged {frac:.3f}", flush=True)
-    assert rec > 0.9
-    N = 32
+    assert rec > 0.95
+    del X, T
+    N = 64
     g = acx.Grid(n=(N, N, N), h=1.0 / N, gamma_hi=0.85, gamma_lo=0.85)
     glin = acx.Grid(n=(N, N, N), h=1.0 / N)
     U0 = torch.from_numpy(learn.steep_pulse_field(g, 999)).cuda()
@@ -197,5 +204,32 @@ def test_trained_indicator_gain(acx, torch_gpu):
     e_lin = acx.diff_f32c(lin.hi, lin.lo, ref)[0]
     print(f"[case] flagged {np.mean(fracs):.3f}, rel L2 vs all-WENO FP64: learned {e_sel:.2e}, linear {e_lin:.2e}",
           flush=True)
-    assert np.mean(fracs) < 0.5
-    assert e_sel < 0.2 * e_lin
+    assert np.mean(fracs) < 0.3
+    assert e_sel < 0.05 * e_lin
+
+
+def test_tile_patch_equals_full_selective_kernel(acx, torch_gpu, monkeypatch):
+    """The selective step (linear pass + CWENO on flagged tiles) equals the
+    CWENO kernel run over every tile with the same flags, to FP32 rounding."""
+    torch = torch_gpu
+    from acoustic_petsc_amd import learn
+    N = 40
+    g = acx.Grid(n=(N, N + 4, N - 4), h=1.0 / N, gamma_hi=0.85, gamma_lo=0.85)
+    U = torch.from_numpy(learn.steep_pulse_field(g, 21)).cuda()
+    rng = np.random.default_rng(4)
+    fl = np.zeros(g.ncells, np.uint8)
+    idx = rng.choice(g.ncells, g.ncells // 50, replace=False)
+    fl[idx] = rng.integers(1, 16, idx.size)
+    flags = torch.from_numpy(fl).cuda()
+    dt = acx.cfl_dt(g, 0.4)
+    a, b = acx.CompensatedState(g, U), acx.CompensatedState(g, U)
+    for _ in range(3):
+        a.step(dt, flags=flags)
+    monkeypatch.setenv("ACX_SEL_FULL", "1")
+    for _ in range(3):
+        b.step(dt, flags=flags)
+    monkeypatch.delenv("ACX_SEL_FULL")
+    ra, rb = a.value64().cpu().numpy(), b.value64().cpu().numpy()
+    err = np.linalg.norm(ra - rb) / np.linalg.norm(rb)
+    print(f"[case] tile patch vs full selective kernel: {err:.2e}", flush=True)
+    assert err < 1e-6
