@@ -58,7 +58,7 @@ __device__ __forceinline__ void features19(const float (&a)[3][5], float (&f)[AC
             d[ax][k] = a[ax][k + 1] - a[ax][k];
             S = fmaxf(S, fabsf(d[ax][k]));
         }
-    const float inv = S > 0.f ? 1.f / S : 0.f;
+    const float inv = S > 0.f ? __builtin_amdgcn_rcpf(S) : 0.f;
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
 #pragma unroll
@@ -66,22 +66,25 @@ __device__ __forceinline__ void features19(const float (&a)[3][5], float (&f)[AC
         f[6 * ax + 4] = (d[ax][1] - d[ax][0]) * inv;
         f[6 * ax + 5] = (d[ax][3] - d[ax][2]) * inv;
     }
-    f[18] = fminf(fmaxf((log10f(S + 1e-30f) + 6.f) / 6.f, -1.f), 2.f);
+    // log10(S) = log2(S) * log10(2)
+    f[18] = fminf(fmaxf((__log2f(S + 1e-30f) * 0.30102999566f + 6.f) * (1.f / 6.f), -1.f), 2.f);
 }
 
+// The 5-point arms of cell (i, j, k), component v, on a periodic grid:
+// a[axis][q] = value at offset q - 2 (32-bit index math; grids stay below
+// 2^32 / 4 values).
 __device__ __forceinline__ void gather_arms(const float *__restrict__ U, const float *__restrict__ D, int nx, int ny,
                                             int nz, int i, int j, int k, int v, float (&a)[3][5])
 {
     auto W = [](int q, int n) { return q < 0 ? q + n : (q >= n ? q - n : q); };
-    auto at = [&](int x, int y, int z) {
-        const size_t idx = (((size_t)W(z, nz) * ny + W(y, ny)) * nx + W(x, nx)) * 4 + v;
-        return D ? U[idx] + D[idx] : U[idx];
-    };
+    const unsigned sx = 4u, sy = 4u * (unsigned)nx, sz = 4u * (unsigned)nx * (unsigned)ny;
+    const unsigned base = (unsigned)k * sz + (unsigned)j * sy + (unsigned)i * sx + (unsigned)v;
+    auto at = [&](unsigned idx) { return D ? U[idx] + D[idx] : U[idx]; };
 #pragma unroll
     for (int o = -2; o <= 2; ++o) {
-        a[0][o + 2] = at(i + o, j, k);
-        a[1][o + 2] = at(i, j + o, k);
-        a[2][o + 2] = at(i, j, k + o);
+        a[0][o + 2] = at(base + (unsigned)(W(i + o, nx) - i) * sx);
+        a[1][o + 2] = at(base + (unsigned)(W(j + o, ny) - j) * sy);
+        a[2][o + 2] = at(base + (unsigned)(W(k + o, nz) - k) * sz);
     }
 }
 
@@ -220,11 +223,12 @@ __global__ void __launch_bounds__(64 * IND_WAVES) k_indicator(const MlpPacked *_
     const size_t nw = (size_t)gridDim.x * IND_WAVES;
     for (size_t c0 = ((size_t)blockIdx.x * IND_WAVES + wv) * 16; c0 < ncell; c0 += nw * 16) {
         {
-            const size_t cell = c0 + (lane >> 2);
+            const unsigned cell = (unsigned)c0 + (unsigned)(lane >> 2);
             const int v = lane & 3;
             float f[ACX_MLP_NFEAT];
             if (cell < ncell) {
-                const int i = (int)(cell % nx), j = (int)((cell / nx) % ny), k = (int)(cell / ((size_t)nx * ny));
+                const unsigned row = cell / (unsigned)nx;
+                const int i = (int)(cell - row * (unsigned)nx), j = (int)(row % (unsigned)ny), k = (int)(row / (unsigned)ny);
                 float a[3][5];
                 gather_arms(HI, D, nx, ny, nz, i, j, k, v, a);
                 features19(a, f);
@@ -380,8 +384,9 @@ __global__ void k_dilate(const uint8_t *__restrict__ in, uint8_t *__restrict__ o
     const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
     auto W = [](int q, int n) { return q < 0 ? q + n : (q >= n ? q - n : q); };
     auto at = [&](int x, int y, int z) { return in[((size_t)W(z, nz) * ny + W(y, ny)) * nx + W(x, nx)]; };
-    out[c] = at(i, j, k) | at(i - 1, j, k) | at(i + 1, j, k) | at(i, j - 1, k) | at(i, j + 1, k) | at(i, j, k - 1) |
-             at(i, j, k + 1);
+    const uint8_t any = at(i, j, k) | at(i - 1, j, k) | at(i + 1, j, k) | at(i, j - 1, k) | at(i, j + 1, k) |
+                        at(i, j, k - 1) | at(i, j, k + 1);
+    out[c] = any ? 0xF : 0;   // the four acoustic components move together
 }
 
 __global__ void k_count(const uint8_t *__restrict__ f, size_t n, unsigned long long *__restrict__ out)
@@ -401,6 +406,8 @@ static int check_ac4(const acx_grid *g)
         if (g->n[d] < 5) return set_error(ACX_EINVAL, "learned indicator: at least 5 cells per axis");
         if (g->bc[d] != ACX_PERIODIC) return set_error(ACX_EUNSUPPORTED, "learned indicator: periodic grids only");
     }
+    if ((size_t)g->n[0] * g->n[1] * g->n[2] * 4 > 0xffffffffull)
+        return set_error(ACX_EUNSUPPORTED, "learned indicator: more than 2^30 cells");
     return ACX_OK;
 }
 

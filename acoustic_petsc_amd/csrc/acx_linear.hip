@@ -20,7 +20,8 @@
 
 namespace acx {
 
-// PF = 0: centre-plane march; PF >= 1: lead-plane march with loads PF planes ahead.
+// PF = 0: centre-plane march (PF = -1: with late epilogue loads); PF >= 1:
+// lead-plane march with loads PF planes ahead.
 template <typename T, int SYS, int BX, int BY, int EPI, int PF, int WPE>
 __global__ void __launch_bounds__(BX *BY) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_linear(LinArgs p)
@@ -29,8 +30,8 @@ k_linear(LinArgs p)
     __shared__ T lds[2][Tile::NS][BY + 6][BX + 6];
     const int tile = xcd_logical_tile(blockIdx.x, gridDim.x);
     const int x0 = (tile % p.ntx) * BX, y0 = ((tile / p.ntx) % p.nty) * BY, zb = (tile / (p.ntx * p.nty)) * p.zc;
-    if constexpr (PF == 0)
-        linear_tile_centre<T, SYS, BX, BY, EPI>(p, x0, y0, zb, lds);
+    if constexpr (PF <= 0)
+        linear_tile_centre<T, SYS, BX, BY, EPI, (PF < 0)>(p, x0, y0, zb, lds);
     else
         linear_tile<T, SYS, BX, BY, EPI, PF>(p, x0, y0, zb, lds);
 }
@@ -54,7 +55,8 @@ static int launch_linear_v(const LinArgs &a, hipStream_t s)
 // plane ahead (0: 2 waves per SIMD, 1: registers capped for 4 waves per
 // SIMD), 2 planes ahead (2) or 3 planes ahead (3); tile 32 x 16 with loads 2
 // (4), 1 (5) or 3 (6) planes ahead; centre-plane march, tile 64 x 8 (7) or
-// 32 x 16 (8), centre-plane 64 x 8 capped for 4 waves per SIMD (9).
+// 32 x 16 (8), centre-plane 64 x 8 capped for 4 waves per SIMD (9); centre-plane
+// 64 x 8 with late epilogue loads (10), and capped for 4 waves per SIMD (11).
 template <typename T, int SYS, int EPI>
 static int launch_linear_t(const LinArgs &a, hipStream_t s)
 {
@@ -69,6 +71,8 @@ static int launch_linear_t(const LinArgs &a, hipStream_t s)
     case 7: return launch_linear_v<T, SYS, EPI, 0, 1, 64, 8>(a, s);
     case 8: return launch_linear_v<T, SYS, EPI, 0, 1, 32, 16>(a, s);
     case 9: return launch_linear_v<T, SYS, EPI, 0, 4, 64, 8>(a, s);
+    case 10: return launch_linear_v<T, SYS, EPI, -1, 1, 64, 8>(a, s);
+    case 11: return launch_linear_v<T, SYS, EPI, -1, 4, 64, 8>(a, s);
     default: return launch_linear_v<T, SYS, EPI, 2, 1, 64, 8>(a, s);
     }
 }

@@ -286,7 +286,7 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
 // halo loads one plane ahead of the neighbours' interior loads.  The column
 // (4 planes ahead) and halo (1 plane ahead) loads keep their raw words until
 // the end of the iteration, so their latency overlaps the iteration's work.
-template <typename T, int SYS, int BX, int BY, int EPI>
+template <typename T, int SYS, int BX, int BY, int EPI, bool LATE_EPI = false>
 __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int y0, int zb,
                                                    T (&lds)[2][LinTile<T, SYS, BX, BY>::NS][BY + 6][BX + 6])
 {
@@ -382,7 +382,7 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
         if (hx >= 0) issue(hn, hcol, z + 1);
         const size_t idx = (size_t)z * plane + col;
         Cell<T, NV> e0, e1, e2;
-        if (active) {
+        auto epi_loads = [&]() {
             if constexpr (EPI == EPI_COMB) e0 = load_cell<T, NV>(U0, idx);
             if constexpr (EPI == EPI_F32C) {
                 if (has_d) e0 = load_cell<T, NV>(D, idx);
@@ -391,7 +391,10 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
                     e2 = load_cell<T, NV>(Q, idx);
                 }
             }
-        }
+        };
+        // epilogue operands: issued with the plane loads, or just before use
+        // (fewer live registers across the stencil)
+        if (!LATE_EPI && active) epi_loads();
 #pragma unroll
         for (int v = 0; v < NS; ++v) lds[buf][v][ty + 3][tx + 3] = qz[3].v[v];
         if (hx >= 0) {
@@ -442,6 +445,7 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
         }
 
         if (active) {
+            if (LATE_EPI) epi_loads();
             Cell<T, NV> o;
             if constexpr (EPI == EPI_RHS) {
                 o = r;

@@ -218,7 +218,9 @@ def evaluate(packed, X, T, thr):
 
 class SelectiveStepper:
     """FP32 compensated SSP-RK3 with the learned indicator: flags from the
-    MLP every `refresh` steps, dilated by one cell, full CWENO4 where set."""
+    MLP every `refresh` steps, dilated by `dilate` cells and spread over the
+    four components (a cell flagged for any component is flagged for all),
+    full CWENO4 where set."""
 
     def __init__(self, grid, packed, U64, thr=0.3, refresh=1, dilate=1):
         import torch

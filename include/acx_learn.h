@@ -68,7 +68,11 @@ int acx_mlp_adam(float *params, const float *grad, float *m, float *v, int step,
 int acx_indicator_flags(const acx_grid *g, const float *HI, const float *D, const void *packed, float thr,
                         uint8_t *flags, void *stream);
 
-/* flags_out[cell] = OR of flags_in over the cell and its 6 face neighbours. */
+/* flags_out[cell] = all four component bits (0xF) if any bit of flags_in is
+ * set over the cell and its 6 face neighbours, else 0.  The components are
+ * spread together because the system couples them: within one step a steep
+ * pressure front makes the velocity steep where it was zero (a flag taken at
+ * the step start from u = 0 would miss it). */
 int acx_flags_dilate(const acx_grid *g, const uint8_t *flags_in, uint8_t *flags_out, void *stream);
 
 /* Number of set bits over all cells (host result). */

@@ -143,6 +143,29 @@ def test_fused_indicator_equals_features_plus_forward(acx, torch_gpu):
     assert np.all((fl == want) | near)
 
 
+def test_flags_dilate(acx, torch_gpu):
+    """Dilation: OR over the cell and its 6 periodic face neighbours, spread
+    to all four component bits."""
+    torch = torch_gpu
+    from acoustic_petsc_amd import learn
+    import ctypes as C
+    g = acx.Grid(n=(9, 7, 5), h=0.1)
+    rng = np.random.default_rng(8)
+    fl = np.zeros(g.ncells, np.uint8)
+    idx = rng.choice(g.ncells, 6, replace=False)
+    fl[idx] = rng.integers(1, 16, idx.size)
+    src = torch.from_numpy(fl).cuda()
+    dst = torch.empty_like(src)
+    L = learn._bind()
+    _, gp = learn._g(g)
+    acx._check(L.acx_flags_dilate(gp, C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), acx._stream()))
+    a = fl.reshape(5, 7, 9) != 0
+    want = a.copy()
+    for ax in range(3):
+        want |= np.roll(a, 1, axis=ax) | np.roll(a, -1, axis=ax)
+    assert np.array_equal(dst.cpu().numpy().reshape(5, 7, 9), np.where(want, 15, 0).astype(np.uint8))
+
+
 def test_selective_limits(acx, torch_gpu):
     """All flags set == full CWENO4 FP32 step, no flag set == the linear scheme
     (both to FP32 rounding: the selective kernel is another instantiation)."""
