@@ -220,9 +220,13 @@ class SelectiveStepper:
     """FP32 compensated SSP-RK3 with the learned indicator: flags from the
     MLP every `refresh` steps, dilated by `dilate` cells and spread over the
     four components (a cell flagged for any component is flagged for all),
-    full CWENO4 where set."""
+    full CWENO4 where set.
 
-    def __init__(self, grid, packed, U64, thr=0.3, refresh=1, dilate=1):
+    Defaults: at CFL 0.4 a front moves 0.4/3 cell per step, so flags taken
+    every 4 steps and dilated by 2 cells still cover it (0.53 cell) with a
+    margin; the indicator then runs once per 4 steps."""
+
+    def __init__(self, grid, packed, U64, thr=0.3, refresh=4, dilate=2):
         import torch
         self.grid, self.packed, self.thr, self.refresh, self.dilate = grid, packed, thr, refresh, dilate
         self.state = CompensatedState(grid, U64)

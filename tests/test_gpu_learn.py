@@ -195,9 +195,10 @@ def test_selective_limits(acx, torch_gpu):
 
 def test_trained_indicator_gain(acx, torch_gpu):
     """Train on solver-labelled 64^3 snapshots (other seeds), then run a
-    held-out steep pulse: the learned stepper must skip most of the WENO work
-    (the method without the indicator evaluates it everywhere) and stay far
-    closer to all-WENO FP64 than the linear scheme does."""
+    held-out steep pulse at 128^3: the learned stepper must skip most of the
+    WENO work (the method without the indicator evaluates it everywhere),
+    stay at the FP32 rounding level of the all-WENO FP64 solution, and far
+    below the error of the linear scheme."""
     torch = torch_gpu
     from acoustic_petsc_amd import learn
     X, T = learn.make_dataset(n_snapshots=4, N=64, seed0=300)
@@ -208,17 +209,17 @@ def test_trained_indicator_gain(acx, torch_gpu):
     print(f"[case] train recall {rec:.3f} precision {prec:.3f} flagged {frac:.3f}", flush=True)
     assert rec > 0.95
     del X, T
-    N = 64
+    N = 128
     g = acx.Grid(n=(N, N, N), h=1.0 / N, gamma_hi=0.85, gamma_lo=0.85)
     glin = acx.Grid(n=(N, N, N), h=1.0 / N)
     U0 = torch.from_numpy(learn.steep_pulse_field(g, 999)).cuda()
     dt = acx.cfl_dt(g, 0.4)
     ref = U0.clone()
     w = (torch.empty_like(ref), torch.empty_like(ref))
-    sel = learn.SelectiveStepper(g, packed, U0, thr=0.3)
+    sel = learn.SelectiveStepper(g, packed, U0)
     lin = acx.CompensatedState(glin, U0)
     fracs = []
-    for _ in range(10):
+    for _ in range(12):
         acx.ssp_rks3_step(g, ref, dt, w)
         sel.step(dt)
         lin.step(dt)
@@ -227,8 +228,9 @@ def test_trained_indicator_gain(acx, torch_gpu):
     e_lin = acx.diff_f32c(lin.hi, lin.lo, ref)[0]
     print(f"[case] flagged {np.mean(fracs):.3f}, rel L2 vs all-WENO FP64: learned {e_sel:.2e}, linear {e_lin:.2e}",
           flush=True)
-    assert np.mean(fracs) < 0.3
-    assert e_sel < 0.05 * e_lin
+    assert np.mean(fracs) < 0.15
+    assert e_sel < 1e-6
+    assert e_sel < 1e-3 * e_lin
 
 
 def test_tile_patch_equals_full_selective_kernel(acx, torch_gpu, monkeypatch):
