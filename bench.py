@@ -93,29 +93,53 @@ def time_steps(torch, step, steps, warmup):
     return e0.elapsed_time(e1) / 1e3, wall
 
 
+def _host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(acx, n_sample=96, max_steps=8, budget_s=10.0):
     """The FP64 oracle (reference loop structure: ghosted copy, per-Gauss-point
     scratch, per-Gauss-point LLF) timed on this host on a 96^3 sample of the
-    same workload."""
+    same workload, on all available cores (up to 16) and on one core."""
     from oracle import oracle as orc
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(threads, 16))
-    orc.lib().orc_set_threads(threads)
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     grid = acx.Grid(n=(n_sample,) * 3, h=1.0 / n_sample, system=acx.ACOUSTIC4)
-    U = acx.field_gaussians(grid, 2025, npulse=4, amp=(1e-4, 1e-3), sigma=(0.02, 0.08))
+    U0 = acx.field_gaussians(grid, 2025, npulse=4, amp=(1e-4, 1e-3), sigma=(0.02, 0.08))
     g = orc.grid3d(n_sample, grid.h, orc.ACOUSTIC4)
     dt = 0.4 * grid.h / 3.0
-    steps, t0 = 0, time.perf_counter()
-    while steps < max_steps:
-        orc.ts_solve("orc_rhs3d_cb_faithful", g, U, dt, dt, 0, 4)
-        steps += 1
-        if time.perf_counter() - t0 >= budget_s:
-            break
-    el = time.perf_counter() - t0
-    return {"value": n_sample ** 3 * STAGES * steps / el, "unit": "cell-updates/s", "cores": threads,
+
+    def run(threads, steps_cap, budget, Uinit, gg, n):
+        orc.lib().orc_set_threads(threads)
+        U = Uinit.copy()
+        steps, t0 = 0, time.perf_counter()
+        while steps < steps_cap:
+            orc.ts_solve("orc_rhs3d_cb_faithful", gg, U, dt * n_sample / n, dt * n_sample / n, 0, 4)
+            steps += 1
+            if time.perf_counter() - t0 >= budget:
+                break
+        el = time.perf_counter() - t0
+        return n ** 3 * STAGES * steps / el, steps, el
+
+    threads = max(1, min(avail, 16))
+    v, steps, el = run(threads, max_steps, budget_s, U0, g, n_sample)
+    # one core on a 64^3 sample of the same workload (a 96^3 step takes ~10 s on one core)
+    n1 = 64
+    grid1 = acx.Grid(n=(n1,) * 3, h=1.0 / n1, system=acx.ACOUSTIC4)
+    U1 = acx.field_gaussians(grid1, 2025, npulse=4, amp=(1e-4, 1e-3), sigma=(0.02, 0.08))
+    v1, steps1, el1 = run(1, 3, budget_s, U1, orc.grid3d(n1, grid1.h, orc.ACOUSTIC4), n1)
+    return {"value": v, "unit": "cell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"{n_sample}^3 ACOUSTIC4 FP64, same 4-Gaussian workload, {steps} SSP-RK3 steps "
-                      f"({STAGES * steps} RHS) of the oracle's reference-structured RHS, {el:.1f} s"}
+                      f"({STAGES * steps} RHS) of the oracle's reference-structured RHS, {el:.1f} s",
+            "single_core": {"value": v1, "sample": f"{n1}^3, {steps1} steps, {el1:.1f} s"},
+            "host": {"cpu": _host_cpu_model(), "available_cores": avail, "nproc": os.cpu_count()}}
 
 
 VARIANTS = ("fp64_cweno", "fp32c", "learned", "ensemble")
