@@ -17,6 +17,7 @@
 #include <math.h>
 #include <time.h>
 #include <sys/time.h>
+#include <string>
 #include <vector>
 #include <hip/hip_runtime.h>
 #include "../../include/acoustic.h"
@@ -48,6 +49,7 @@ struct _p_DM {
     Vec coords;      // coordinate vector (global)
     bool is_coord;
     std::vector<Vec> local_pool;
+    std::vector<std::string> fields;   // DMDASetFieldName
 };
 
 struct _p_Vec {
@@ -296,8 +298,10 @@ PetscErrorCode DMDestroy(DM *da)
 
 PetscErrorCode DMDASetFieldName(DM da, PetscInt nf, const char *name)
 {
-    (void)name;
-    return (nf >= 0 && nf < da->dof) ? 0 : PetscLiteError(63, "field index out of range");
+    if (nf < 0 || nf >= da->dof) return PetscLiteError(63, "field index out of range");
+    if ((int)da->fields.size() < da->dof) da->fields.resize(da->dof);
+    da->fields[nf] = name ? name : "";
+    return 0;
 }
 
 PetscErrorCode DMDAGetCorners(DM da, PetscInt *xs, PetscInt *ys, PetscInt *zs, PetscInt *xm, PetscInt *ym,
@@ -536,7 +540,201 @@ PetscErrorCode VecLiteGetDeviceArray(Vec v, double **d)
     return 0;
 }
 
-// ---- TS ----------------------------------------------------------------------
+// ---- Viewers ---------------------------------------------------------------
+
+} // extern "C"
+
+struct _p_PetscViewer {
+    FILE *fp;
+    bool binary;
+    PetscFileMode mode;
+    PetscViewerFormat fmt;
+    std::string name;
+};
+
+static void put_be32(FILE *fp, int32_t v)
+{
+    const uint32_t u = (uint32_t)v;
+    const unsigned char b[4] = {(unsigned char)(u >> 24), (unsigned char)(u >> 16), (unsigned char)(u >> 8),
+                                (unsigned char)u};
+    fwrite(b, 1, 4, fp);
+}
+
+static bool get_be32(FILE *fp, int32_t *v)
+{
+    unsigned char b[4];
+    if (fread(b, 1, 4, fp) != 4) return false;
+    *v = (int32_t)(((uint32_t)b[0] << 24) | ((uint32_t)b[1] << 16) | ((uint32_t)b[2] << 8) | (uint32_t)b[3]);
+    return true;
+}
+
+static uint64_t bswap64(uint64_t x)
+{
+    x = ((x & 0x00000000FFFFFFFFull) << 32) | (x >> 32);
+    x = ((x & 0x0000FFFF0000FFFFull) << 16) | ((x >> 16) & 0x0000FFFF0000FFFFull);
+    return ((x & 0x00FF00FF00FF00FFull) << 8) | ((x >> 8) & 0x00FF00FF00FF00FFull);
+}
+
+static PetscErrorCode viewer_open(const char *name, bool binary, PetscFileMode mode, PetscViewer *out)
+{
+    if (!name || !out) return PetscLiteError(85, "viewer: null argument");
+    const char *m = binary ? (mode == FILE_MODE_READ ? "rb" : mode == FILE_MODE_APPEND ? "ab" : "wb")
+                           : (mode == FILE_MODE_APPEND ? "a" : "w");
+    FILE *fp = fopen(name, m);
+    if (!fp) return PetscLiteError(65, "cannot open %s", name);
+    PetscViewer v = new _p_PetscViewer();
+    v->fp = fp;
+    v->binary = binary;
+    v->mode = mode;
+    v->fmt = PETSC_VIEWER_DEFAULT;
+    v->name = name;
+    *out = v;
+    return 0;
+}
+
+extern "C" {
+
+PetscErrorCode PetscViewerBinaryOpen(MPI_Comm comm, const char *name, PetscFileMode mode, PetscViewer *viewer)
+{
+    (void)comm;
+    return viewer_open(name, true, mode, viewer);
+}
+
+PetscErrorCode PetscViewerASCIIOpen(MPI_Comm comm, const char *name, PetscViewer *viewer)
+{
+    (void)comm;
+    return viewer_open(name, false, FILE_MODE_WRITE, viewer);
+}
+
+PetscErrorCode PetscViewerPushFormat(PetscViewer viewer, PetscViewerFormat format)
+{
+    if (!viewer) return PetscLiteError(85, "viewer: null");
+    viewer->fmt = format;
+    return 0;
+}
+
+PetscErrorCode PetscViewerPopFormat(PetscViewer viewer)
+{
+    if (!viewer) return PetscLiteError(85, "viewer: null");
+    viewer->fmt = PETSC_VIEWER_DEFAULT;
+    return 0;
+}
+
+PetscErrorCode PetscViewerDestroy(PetscViewer *viewer)
+{
+    if (!viewer || !*viewer) return 0;
+    if ((*viewer)->fp) fclose((*viewer)->fp);
+    delete *viewer;
+    *viewer = nullptr;
+    return 0;
+}
+
+PetscErrorCode DMView(DM da, PetscViewer viewer)
+{
+    if (!da || !viewer) return PetscLiteError(85, "DMView: null argument");
+    if (viewer->binary || viewer->fmt != PETSC_VIEWER_ASCII_VTK) {
+        if (!viewer->binary)
+            fprintf(viewer->fp, "DM Object: 1 MPI process\n  type: da\nDMDA %d x %d x %d, dof %d, stencil width %d\n",
+                    da->M[0], da->dim > 1 ? da->M[1] : 1, da->dim > 2 ? da->M[2] : 1, da->dof, da->s);
+        return 0;
+    }
+    const int nx = da->M[0], ny = da->dim > 1 ? da->M[1] : 1, nz = da->dim > 2 ? da->M[2] : 1;
+    const size_t np = (size_t)nx * ny * nz;
+    FILE *fp = viewer->fp;
+    fprintf(fp, "# vtk DataFile Version 2.0\nGenerated by acoustic_petsc_amd petsclite\nASCII\n");
+    fprintf(fp, "DATASET STRUCTURED_GRID\nDIMENSIONS %d %d %d\nPOINTS %zu double\n", nx, ny, nz, np);
+    const double *c = nullptr;
+    if (da->coords) {
+        PetscErrorCode e = to_host(da->coords);
+        if (e) return e;
+        c = da->coords->host;
+    }
+    for (size_t q = 0; q < np; ++q) {
+        const size_t i = q % nx, j = (q / nx) % ny, k = q / ((size_t)nx * ny);
+        double p[3] = {(double)i, (double)j, (double)k};
+        if (c)
+            for (int d = 0; d < da->dim; ++d) p[d] = c[q * da->dim + d];
+        fprintf(fp, "%.17g %.17g %.17g\n", p[0], p[1], p[2]);
+    }
+    fprintf(fp, "POINT_DATA %zu\n", np);
+    return 0;
+}
+
+PetscErrorCode VecView(Vec v, PetscViewer viewer)
+{
+    if (!v || !viewer) return PetscLiteError(85, "VecView: null argument");
+    if (v->local) return PetscLiteError(56, "VecView: global vectors only");
+    PetscErrorCode e = to_host(v);
+    if (e) return e;
+    FILE *fp = viewer->fp;
+    if (viewer->binary) {
+        if (viewer->mode == FILE_MODE_READ) return PetscLiteError(56, "VecView: viewer opened for reading");
+        if (v->n > 0x7fffffff) return PetscLiteError(56, "VecView: vector longer than 2^31 - 1");
+        put_be32(fp, VEC_FILE_CLASSID);
+        put_be32(fp, (int32_t)v->n);
+        std::vector<uint64_t> buf(v->n);
+        for (size_t q = 0; q < v->n; ++q) {
+            uint64_t u;
+            memcpy(&u, &v->host[q], 8);
+            buf[q] = bswap64(u);
+        }
+        if (fwrite(buf.data(), 8, v->n, fp) != v->n) return PetscLiteError(66, "VecView: write failed");
+        fflush(fp);
+        const int dof = v->dm ? v->dm->dof : 1;
+        if (dof > 1) {
+            FILE *info = fopen((viewer->name + ".info").c_str(), "w");
+            if (info) {
+                fprintf(info, "-vecload_block_size %d\n", dof);
+                fclose(info);
+            }
+        }
+        return 0;
+    }
+    if (viewer->fmt == PETSC_VIEWER_ASCII_VTK && v->dm) {
+        const int dof = v->dm->dof;
+        std::string name;
+        for (int f = 0; f < dof; ++f) {
+            const bool named = f < (int)v->dm->fields.size() && !v->dm->fields[f].empty();
+            name += (f ? "_" : "") + (named ? v->dm->fields[f] : std::string("comp") + std::to_string(f));
+        }
+        fprintf(fp, "SCALARS %s double %d\nLOOKUP_TABLE default\n", name.c_str(), dof);
+        const size_t np = v->n / dof;
+        for (size_t q = 0; q < np; ++q) {
+            for (int f = 0; f < dof; ++f) fprintf(fp, f ? " %.17g" : "%.17g", v->host[q * dof + f]);
+            fputc('\n', fp);
+        }
+        return 0;
+    }
+    fprintf(fp, "Vec Object: 1 MPI process\n  type: seq\n");
+    for (size_t q = 0; q < v->n; ++q) fprintf(fp, "%.17g\n", v->host[q]);
+    return 0;
+}
+
+PetscErrorCode VecLoad(Vec v, PetscViewer viewer)
+{
+    if (!v || !viewer) return PetscLiteError(85, "VecLoad: null argument");
+    if (!viewer->binary || viewer->mode != FILE_MODE_READ)
+        return PetscLiteError(56, "VecLoad: needs a binary viewer opened with FILE_MODE_READ");
+    if (v->local) return PetscLiteError(56, "VecLoad: global vectors only");
+    int32_t cls = 0, n = 0;
+    if (!get_be32(viewer->fp, &cls) || !get_be32(viewer->fp, &n))
+        return PetscLiteError(66, "VecLoad: %s: truncated header", viewer->name.c_str());
+    if (cls != VEC_FILE_CLASSID)
+        return PetscLiteError(79, "VecLoad: %s: not a Vec record (class id %d)", viewer->name.c_str(), cls);
+    if ((size_t)n != v->n)
+        return PetscLiteError(60, "VecLoad: %s holds %d values, vector has %zu", viewer->name.c_str(), n, v->n);
+    std::vector<uint64_t> buf(v->n);
+    if (fread(buf.data(), 8, v->n, viewer->fp) != v->n)
+        return PetscLiteError(66, "VecLoad: %s: truncated data", viewer->name.c_str());
+    for (size_t q = 0; q < v->n; ++q) {
+        const uint64_t u = bswap64(buf[q]);
+        memcpy(&v->host[q], &u, 8);
+    }
+    v->state = VEC_HOST;
+    return to_dev(v);
+}
+
+// ---- TS ------------------------------------------------------------------------
 
 PetscErrorCode TSCreate(MPI_Comm comm, TS *ts)
 {

@@ -142,6 +142,27 @@ PetscErrorCode VecGetSize(Vec v, PetscInt *n);
 /* device address of a vector's data (synchronised), for C-ABI calls */
 PetscErrorCode VecLiteGetDeviceArray(Vec v, double **d);
 
+/* ---- Viewers (3d/weno4/plane_wave/hype.c:147-151,190-196, monitor_function.c:22-58) ----
+ * Binary: the PETSc binary format of VecView/VecLoad, one record per call:
+ * int32 VEC_FILE_CLASSID (1211214), int32 length, then the values as float64,
+ * all big-endian, in natural (x fastest, components interlaced) order; a
+ * "<file>.info" text file carries "-vecload_block_size <dof>" for dof > 1.
+ * ASCII + PETSC_VIEWER_ASCII_VTK: legacy VTK "STRUCTURED_GRID": DMView writes
+ * the header and the point coordinates (cell centres from
+ * DMDASetUniformCoordinates, else indices), VecView the POINT_DATA block with
+ * the DMDA field name(s).  Plain ASCII: one value per line. */
+typedef enum { FILE_MODE_READ, FILE_MODE_WRITE, FILE_MODE_APPEND } PetscFileMode;
+typedef enum { PETSC_VIEWER_DEFAULT, PETSC_VIEWER_ASCII_VTK } PetscViewerFormat;
+#define VEC_FILE_CLASSID 1211214
+PetscErrorCode PetscViewerBinaryOpen(MPI_Comm comm, const char *name, PetscFileMode mode, PetscViewer *viewer);
+PetscErrorCode PetscViewerASCIIOpen(MPI_Comm comm, const char *name, PetscViewer *viewer);
+PetscErrorCode PetscViewerPushFormat(PetscViewer viewer, PetscViewerFormat format);
+PetscErrorCode PetscViewerPopFormat(PetscViewer viewer);
+PetscErrorCode PetscViewerDestroy(PetscViewer *viewer);
+PetscErrorCode VecView(Vec v, PetscViewer viewer);
+PetscErrorCode VecLoad(Vec v, PetscViewer viewer);
+PetscErrorCode DMView(DM da, PetscViewer viewer);
+
 /* ---- TS ------------------------------------------------------------------ */
 PetscErrorCode TSCreate(MPI_Comm comm, TS *ts);
 PetscErrorCode TSDestroy(TS *ts);

@@ -76,3 +76,48 @@ def test_fused_generic_and_oracle_agree(tmp_path, orc, acx, gamma):
     e4 = np.linalg.norm(res["rk104"] - ref4) / np.linalg.norm(ref4)
     print(f"[case] gamma={gamma} rk104 vs oracle rel {e4:.3e}", flush=True)
     assert e4 < 1e-12
+
+
+def test_restart_files_and_vtk(tmp_path):
+    """Monitor output and restart of the reference drivers
+    (3d/weno4/plane_wave/monitor_function.c:12-62, hype.c:144-196): a run
+    resumed from restart1.bin (VecLoad) ends bitwise where the uninterrupted
+    run ends; the binary files follow the PETSc layout (class id 1211214,
+    int32 length, big-endian float64); the ASCII VTK files hold the grid and
+    the final state."""
+    N, steps, ri, wi = 20, 10, 4, 5
+    (tmp_path / "plot").mkdir()
+    out = run([os.path.join(DRV, "restart_3d"), str(N), str(steps), str(ri), str(wi), "run", str(tmp_path)])
+    assert "10 t = " in out
+    run([os.path.join(DRV, "restart_3d"), str(N), str(steps), str(ri), str(wi), "resume", str(tmp_path)])
+    a = (tmp_path / "final.bin").read_bytes()
+    b = (tmp_path / "final_resumed.bin").read_bytes()
+    assert a == b
+    n = N ** 3 * 4
+    for name in ("final.bin", "restart1.bin", "restart2.bin"):
+        raw = (tmp_path / name).read_bytes()
+        assert len(raw) == 8 + 8 * n
+        head = np.frombuffer(raw[:8], dtype=">i4")
+        assert head[0] == 1211214 and head[1] == n
+    assert (tmp_path / "final.bin.info").read_text().strip() == "-vecload_block_size 4"
+    final = np.frombuffer(a[8:], dtype=">f8").astype(np.float64)
+    r1 = np.frombuffer((tmp_path / "restart1.bin").read_bytes()[8:], dtype=">f8")
+    r2 = np.frombuffer((tmp_path / "restart2.bin").read_bytes()[8:], dtype=">f8")
+    assert np.abs(r2).max() > 0 and not np.array_equal(r1, r2)   # step 0 vs step 8
+    # VTK written every 5 steps and at the end
+    for s in (0, 5, 10):
+        assert (tmp_path / "plot" / f"sol-{s:05d}.vtk").exists()
+    lines = (tmp_path / "plot" / f"sol-{steps:08d}.vtk").read_text().splitlines()
+    assert lines[0] == "# vtk DataFile Version 2.0" and lines[2] == "ASCII"
+    assert lines[3] == "DATASET STRUCTURED_GRID" and lines[4] == f"DIMENSIONS {N} {N} {N}"
+    assert lines[5] == f"POINTS {N ** 3} double"
+    pts = np.array([[float(t) for t in ln.split()] for ln in lines[6:6 + N ** 3]])
+    h = 1.0 / N
+    i = np.arange(N ** 3)
+    want = np.stack([(i % N + 0.5) * h, ((i // N) % N + 0.5) * h, (i // N ** 2 + 0.5) * h], 1)
+    assert np.max(np.abs(pts - want)) < 1e-14
+    k = 6 + N ** 3
+    assert lines[k] == f"POINT_DATA {N ** 3}" and lines[k + 1] == "SCALARS P_U_V_W double 4"
+    assert lines[k + 2] == "LOOKUP_TABLE default"
+    vals = np.array([[float(t) for t in ln.split()] for ln in lines[k + 3:k + 3 + N ** 3]]).ravel()
+    assert np.array_equal(vals, final)
