@@ -140,6 +140,8 @@ static int stage32c(const acx_grid *g, const float *HI, const float *D, const fl
 
 using namespace acx;
 
+__global__ void k_point_add(double *x, double v) { *x += v; }
+
 extern "C" {
 
 const char *acx_last_error(void) { return g_err; }
@@ -204,6 +206,39 @@ int acx_ssp_rks3_step_f32c(const acx_grid *g, const float *HI, const float *LO, 
     if ((rc = stage32c(g, HI, D0, LO, D1, nullptr, 1.0, dt / 2, 0, nullptr, s))) return rc;
     if ((rc = stage32c(g, HI, D1, LO, D0, nullptr, 1.0 / 3.0, dt / 6, 0, nullptr, s))) return rc;
     return stage32c(g, HI, D0, LO, HI_OUT, LO_OUT, 1.0, dt / 2, 1, nullptr, s);
+}
+
+
+static int point_index(const acx_grid *g, int i, int j, int k, int comp, size_t *idx)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (i < 0 || j < 0 || k < 0 || i >= g->n[0] || j >= g->n[1] || k >= g->n[2])
+        return set_error(ACX_EINVAL, "cell (%d, %d, %d) outside the grid", i, j, k);
+    if (comp < 0 || comp >= nvar_of(g)) return set_error(ACX_EINVAL, "component %d out of range", comp);
+    *idx = (((size_t)k * g->n[1] + j) * g->n[0] + i) * nvar_of(g) + comp;
+    return ACX_OK;
+}
+
+int acx_point_add_f64(const acx_grid *g, double *X, int i, int j, int k, int comp, double value, void *stream)
+{
+    size_t idx;
+    int rc = point_index(g, i, j, k, comp, &idx);
+    if (rc) return rc;
+    if (!X) return set_error(ACX_EINVAL, "X is NULL");
+    hipLaunchKernelGGL(k_point_add, dim3(1), dim3(1), 0, (hipStream_t)stream, X + idx, value);
+    return hip_check(hipGetLastError(), "point add");
+}
+
+int acx_point_get_f64(const acx_grid *g, const double *X, int i, int j, int k, int comp, double *out, void *stream)
+{
+    size_t idx;
+    int rc = point_index(g, i, j, k, comp, &idx);
+    if (rc) return rc;
+    if (!X || !out) return set_error(ACX_EINVAL, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    rc = hip_check(hipMemcpyAsync(out, X + idx, sizeof(double), hipMemcpyDeviceToHost, s), "point get");
+    return rc ? rc : hip_check(hipStreamSynchronize(s), "point get");
 }
 
 } // extern "C"

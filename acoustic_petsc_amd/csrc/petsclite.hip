@@ -935,12 +935,37 @@ extern "C" PetscErrorCode AppCtxSetDefaults(AppCtx *c)
     c->rho_0 = 1.0;
     c->left_boundary = c->right_boundary = c->top_boundary = periodic;
     c->bottom_boundary = c->front_boundary = c->back_boundary = periodic;
+    c->source = NULL;
+    c->isx = c->isy = c->isz = 0;
+    c->receiver = NULL;
+    c->irx = c->iry = c->irz = 0;
     return 0;
+}
+
+// receiver sample and point source of one RHS evaluation at time t
+// (3d/weno4/point_source/rhs_function.c:337-346); `scale` multiplies the
+// source (1 for an RHS, the stage's dt coefficient for a fused stage output).
+static PetscErrorCode point_terms_receiver(const AppCtx *c, const acx_grid *g, const double *U, double t)
+{
+    if (!c->receiver) return 0;
+    double val = 0.0;
+    int rc = acx_point_get_f64(g, U, c->irx, c->iry, c->irz, 0, &val, nullptr);
+    if (rc) return PetscLiteError(76 + rc, "receiver: %s", acx_last_error());
+    fprintf(c->receiver, "%e\t%.12e\n", t, val);
+    return 0;
+}
+
+static PetscErrorCode point_terms_source(const AppCtx *c, const acx_grid *g, double *OUT, double t, double scale)
+{
+    if (!c->source) return 0;
+    const double v = scale * c->source(t) / (g->h * g->h * g->h);
+    const int comp = g->system == ACX_WAVE2 ? 1 : 0;   // WAVE2: v_t; ACOUSTIC4: p_t
+    int rc = acx_point_add_f64(g, OUT, c->isx, c->isy, c->isz, comp, v, nullptr);
+    return rc ? PetscLiteError(76 + rc, "point source: %s", acx_last_error()) : 0;
 }
 
 extern "C" PetscErrorCode RHSFunction(TS ts, PetscReal t, Vec U, Vec RHS, void *ctx)
 {
-    (void)t;
     AppCtx *c = (AppCtx *)ctx;
     DM da;
     PetscErrorCode e = TSGetDM(ts, &da);
@@ -951,6 +976,8 @@ extern "C" PetscErrorCode RHSFunction(TS ts, PetscReal t, Vec U, Vec RHS, void *
     if ((e = VecLiteGetDeviceArray(U, &du)) || (e = VecLiteGetDeviceArray(RHS, &dr))) return e;
     int rc = acx_rhs_f64(&g, du, dr, nullptr);
     if (rc) return PetscLiteError(76 + rc, "RHSFunction: %s", acx_last_error());
+    if ((e = point_terms_source(c, &g, dr, t, 1.0))) return e;
+    if ((e = point_terms_receiver(c, &g, du, t))) return e;
     if (g.system == ACX_ACOUSTIC4) {
         // LinearisedEuler/rhs_function.c:382-384 (one rank: the MIN-reduction is the value)
         c->dt = acx_cfl_dt(&g, c->CFL);
@@ -969,7 +996,26 @@ static PetscErrorCode fused_rks3_step(TS ts, Vec u, double dt, bool *done)
     if ((e = VecLiteGetDeviceArray(u, &du)) || (e = VecLiteGetDeviceArray(ts->w0, &d0)) ||
         (e = VecLiteGetDeviceArray(ts->w1, &d1)))
         return e;
-    int rc = acx_ssp_rks3_step_f64(&g, du, d0, d1, dt, nullptr);
+    const AppCtx *c = (const AppCtx *)ts->rhs_ctx;
+    int rc;
+    if (!c->source && !c->receiver) {
+        rc = acx_ssp_rks3_step_f64(&g, du, d0, d1, dt, nullptr);
+    } else {
+        // the four stages of acx_ssp_rks3_step_f64 with the point terms of each
+        // RHS evaluation (stage times t, t + dt/2, t + dt, t + dt/2)
+        const double t = ts->t;
+        struct { const double *q, *u0; double *out, a, b, c, ts; } st[4] = {
+            {du, nullptr, d0, 0.0, 1.0, dt / 2, t},
+            {d0, nullptr, d1, 0.0, 1.0, dt / 2, t + dt / 2},
+            {d1, du, d0, 2.0 / 3.0, 1.0 / 3.0, dt / 6, t + dt},
+            {d0, nullptr, du, 0.0, 1.0, dt / 2, t + dt / 2}};
+        rc = 0;
+        for (int k = 0; k < 4 && !rc; ++k) {
+            if ((e = point_terms_receiver(c, &g, st[k].q, st[k].ts))) return e;
+            rc = acx_stage_f64(&g, st[k].q, st[k].u0, st[k].out, st[k].a, st[k].b, st[k].c, nullptr);
+            if (!rc && (e = point_terms_source(c, &g, st[k].out, st[k].ts, st[k].c))) return e;
+        }
+    }
     if (rc) return PetscLiteError(76 + rc, "TSSolve fused step: %s", acx_last_error());
     if (g.system == ACX_ACOUSTIC4) ((AppCtx *)ts->rhs_ctx)->dt = acx_cfl_dt(&g, ((AppCtx *)ts->rhs_ctx)->CFL);
     *done = true;

@@ -52,6 +52,17 @@ typedef struct {
     PetscReal gammaHi, gammaLo;    /* reconstruction.c:42-43 (shipped 1, 0; nonlinear 0.85, 0.85) */
     PetscReal wave_speed;          /* c0 (hype.h:39 wave_speed, LinearisedEuler hype.h:37 c_0) */
     PetscReal rho_0;               /* LinearisedEuler hype.h:38 */
+    /* point source and receiver of 3d/weno4/point_source (hype.h AppCtx isx..irz,
+       rhs_function.c:337-346, source.c:11-17).  When source is not NULL, every
+       RHS evaluation at time t adds source(t) / h^3 to component 1 of cell
+       (isx, isy, isz); when receiver is not NULL, every RHS evaluation appends
+       "t<TAB>u" (format "%e\t%.12e\n", component 0 at (irx, iry, irz) of the
+       evaluated state) to it.  The fused TSSolve path does both per stage, with
+       the stage times t, t + dt/2, t + dt, t + dt/2 of TSSSPRKS3. */
+    PetscReal (*source)(PetscReal t);
+    PetscInt isx, isy, isz;
+    FILE *receiver;
+    PetscInt irx, iry, irz;
 } AppCtx;
 
 /* Fills the operator fields with the reference's defaults (shipped linear

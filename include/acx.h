@@ -124,6 +124,16 @@ int acx_field_gaussians(const acx_grid *g, uint64_t seed, int npulse, double ami
  * cell averages at time t (ACOUSTIC4, periodic unit cube). */
 int acx_field_plane_wave(const acx_grid *g, const int k[3], double A, double t, double *U);
 
+/* One entry of a device state: X[cell (i, j, k), component comp] += value.
+ * With X the RHS and value = Source(t) / h^3 on component 1 this is the
+ * point source of the reference's WENO driver
+ * (3d/weno4/point_source/rhs_function.c:337-340). */
+int acx_point_add_f64(const acx_grid *g, double *X, int i, int j, int k, int comp, double value, void *stream);
+
+/* *out = X[cell (i, j, k), component comp] (host result; synchronises the
+ * stream), the receiver read of 3d/weno4/point_source/rhs_function.c:342-346. */
+int acx_point_get_f64(const acx_grid *g, const double *X, int i, int j, int k, int comp, double *out, void *stream);
+
 const char *acx_last_error(void);
 int acx_version(void);
 

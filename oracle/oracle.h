@@ -90,6 +90,19 @@ int orc_ts_solve(orc_rhs_fn f, void *ctx, size_t n, double *u, double t0, double
 
 /* RHS adaptor for orc_ts_solve: ctx is an orc_grid3d*; faithful / streaming. */
 int orc_rhs3d_cb_faithful(void *ctx, double t, const double *U, double *F);
+
+/* WENO point source and receiver (3d/weno4/point_source/rhs_function.c:337-346,
+ * source.c:11-17): the faithful RHS plus Source(t)/h^3 on component 1 of cell
+ * src; every call appends (t, component 0 of U at cell rcv) to the trace. */
+typedef struct {
+    orc_grid3d g;
+    int    src[3], rcv[3];
+    double f0;
+    double *trace_t, *trace_u;   /* caller-owned, capacity cap */
+    int    ntrace, cap;
+} orc_point_source3d;
+double orc_weno_source(double t, double f0);
+int orc_rhs3d_cb_point_source(void *ctx, double t, const double *U, double *F);
 int orc_rhs3d_cb_stream(void *ctx, double t, const double *U, double *F);
 
 /* ---- 3D plane-wave convergence case (3d/weno4/plane_wave) --------------- */

@@ -36,6 +36,11 @@ class CT(C.Structure):
                 ("f0", C.c_double)]
 
 
+class PS(C.Structure):
+    _fields_ = [("g", G3), ("src", C.c_int * 3), ("rcv", C.c_int * 3), ("f0", C.c_double), ("trace_t", dp),
+                ("trace_u", dp), ("ntrace", C.c_int), ("cap", C.c_int)]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB_PATH
@@ -70,6 +75,9 @@ def lib():
         L.orc_standing_wave2d_init.argtypes = [C.c_int, dp]
         L.orc_standing_wave2d_error.argtypes = [C.c_int, C.c_double, dp, dp, dp]
         L.orc_ctcs3d_run.argtypes = [C.POINTER(CT), C.c_int, dp]
+        L.orc_rhs3d_cb_point_source.argtypes = [C.c_void_p, C.c_double, dp, dp]
+        L.orc_weno_source.argtypes = [C.c_double, C.c_double]
+        L.orc_weno_source.restype = C.c_double
         _lib = L
     return _lib
 
@@ -142,3 +150,20 @@ def ctcs_trace(n=(100, 100, 100), dx=0.5, dt=0.0005, c0=250.0, stencil=2, src=(5
     tr = np.zeros(nt)
     lib().orc_ctcs3d_run(C.byref(c), nt, P(tr))
     return tr
+
+
+def point_source_run(N, h, c0, dt, nt, src, rcv, f0=100.0, gamma=(1.0, 0.0)):
+    """The WENO point-source program (3d/weno4/point_source/hype.c:14-183): WAVE2
+    from rest, one direct RHS call at t = 0 (hype.c:170), then nt SSP-RK3 (s = 4)
+    steps.  Returns (state (N,N,N,2), receiver times, receiver values): one
+    receiver sample per RHS evaluation, as rhs_function.c:342-346 writes them."""
+    g = grid3d(N, h, WAVE2, gamma, c0=c0)
+    U = np.zeros((N, N, N, 2))
+    cap = 4 * nt + 16
+    tt, tu = np.zeros(cap), np.zeros(cap)
+    ps = PS(g, tuple(src), tuple(rcv), f0, P(tt), P(tu), 0, cap)
+    F = np.zeros_like(U)
+    if lib().orc_rhs3d_cb_point_source(C.cast(C.pointer(ps), C.c_void_p), 0.0, P(U), P(F)):
+        raise RuntimeError("oracle point-source rhs failed")
+    ts_solve("orc_rhs3d_cb_point_source", ps, U, dt, nt * dt, 0, 4)
+    return U, tt[:ps.ntrace].copy(), tu[:ps.ntrace].copy()

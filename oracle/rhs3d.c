@@ -360,6 +360,34 @@ int orc_rhs3d_cb_faithful(void *ctx, double t, const double *U, double *F)
     return orc_rhs3d_faithful((const orc_grid3d *)ctx, U, F, NULL);
 }
 
+double orc_weno_source(double t, double f0)
+{
+    /* 3d/weno4/point_source/source.c:11-17 */
+    double t0 = 4.0 / f0;
+    return -2.0 * (t - t0) * f0 * f0 * exp(-1.0 * f0 * f0 * (t - t0) * (t - t0));
+}
+
+int orc_rhs3d_cb_point_source(void *ctx, double t, const double *U, double *F)
+{
+    orc_point_source3d *ps = (orc_point_source3d *)ctx;
+    const orc_grid3d *g = &ps->g;
+    const int nv = orc_nvar(g->system);
+    int rc = orc_rhs3d_faithful(g, U, F, NULL);
+    if (rc) return rc;
+    const size_t nx = (size_t)g->N[0], ny = (size_t)g->N[1];
+    /* rhs_function.c:337-340: rhs[isz][isy][isx].comp[1] += Source(t) * r1_h3 */
+    const size_t s = (((size_t)ps->src[2] * ny + ps->src[1]) * nx + ps->src[0]) * nv + 1;
+    F[s] += orc_weno_source(t, ps->f0) / (g->h * g->h * g->h);
+    /* rhs_function.c:342-346: receiver u[irz][iry][irx].comp[0] at every RHS call */
+    if (ps->ntrace < ps->cap) {
+        const size_t r = (((size_t)ps->rcv[2] * ny + ps->rcv[1]) * nx + ps->rcv[0]) * nv;
+        ps->trace_t[ps->ntrace] = t;
+        ps->trace_u[ps->ntrace] = U[r];
+        ps->ntrace++;
+    }
+    return 0;
+}
+
 int orc_rhs3d_cb_stream(void *ctx, double t, const double *U, double *F)
 {
     (void)t;
