@@ -26,7 +26,9 @@ LIB_PATH = _os.path.join(_HERE, "libacx.so")
 WAVE2 = 0
 ACOUSTIC4 = 1
 PERIODIC = 0
-ADIABATIC = 1
+ADIABATIC = 1        # WAVE2 walls: component 0 mirrored with its sign flipped
+REFLECTIVE = 2       # ACOUSTIC4 walls: mirror, normal velocity negated
+TRANSMISSIVE = 3     # ACOUSTIC4 walls: copy of the boundary cell
 
 ACX_ERRORS = {1: "invalid argument", 2: "unsupported", 3: "HIP runtime error", 4: "non-finite state"}
 

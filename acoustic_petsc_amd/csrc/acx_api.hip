@@ -40,17 +40,20 @@ static int check_grid(const acx_grid *g)
     if (!g) return set_error(ACX_EINVAL, "grid is NULL");
     for (int d = 0; d < 3; ++d) {
         if (g->n[d] < 4) return set_error(ACX_EINVAL, "n[%d] = %d: at least 4 cells per axis are required", d, g->n[d]);
-        if (g->bc[d] != ACX_PERIODIC && g->bc[d] != ACX_ADIABATIC)
+        if (g->bc[d] < ACX_PERIODIC || g->bc[d] > ACX_TRANSMISSIVE)
             return set_error(ACX_EINVAL, "bc[%d] = %d is not a boundary kind", d, g->bc[d]);
     }
     if ((size_t)g->n[0] * g->n[1] > 0x7fffffffu) return set_error(ACX_EINVAL, "plane too large");
     if (g->system != ACX_WAVE2 && g->system != ACX_ACOUSTIC4)
         return set_error(ACX_EINVAL, "system %d unknown", g->system);
     if (!(g->h > 0)) return set_error(ACX_EINVAL, "h must be positive");
+    for (int d = 0; d < 3; ++d) {
+        if (g->system == ACX_ACOUSTIC4 && g->bc[d] == ACX_ADIABATIC)
+            return set_error(ACX_EUNSUPPORTED, "ACOUSTIC4: axis %d: adiabatic walls are a WAVE2 rule; use reflective", d);
+        if (g->system == ACX_WAVE2 && (g->bc[d] == ACX_REFLECTIVE || g->bc[d] == ACX_TRANSMISSIVE))
+            return set_error(ACX_EUNSUPPORTED, "WAVE2: axis %d: reflective/transmissive walls are ACOUSTIC4 rules", d);
+    }
     if (g->system == ACX_ACOUSTIC4) {
-        for (int d = 0; d < 3; ++d)
-            if (g->bc[d] != ACX_PERIODIC)
-                return set_error(ACX_EUNSUPPORTED, "ACOUSTIC4 supports periodic boundaries only");
         if (!(g->rho0 > 0) || !(g->c0 > 0)) return set_error(ACX_EINVAL, "rho0 and c0 must be positive");
     }
     if (!(g->gamma_hi > 0 && g->gamma_hi <= 1) || !(g->gamma_lo >= 0 && g->gamma_lo <= 1))

@@ -114,34 +114,31 @@ k_cweno(CwArgs p)
     const int zb = (tile / (p.ntx * p.nty)) * p.zc, ze = min(zb + p.zc, nz);
     const bool interior = tx >= 1 && tx <= BX - 2 && ty >= 1 && ty <= BY - 2 && gx < nx && gy < ny;
     const size_t plane = (size_t)nx * ny;
-    const int perx = p.bc[0] == ACX_PERIODIC, pery = p.bc[1] == ACX_PERIODIC, perz = p.bc[2] == ACX_PERIODIC;
+    const int kx = p.bc[0], ky = p.bc[1], kz = p.bc[2];   // boundary kinds
 
     // own column (mapped) and its ghost sign
     float sx0 = 1.f, sy0 = 1.f;
-    const uint32_t ocol = (uint32_t)(map_axis(gy, ny, pery, sy0) * nx + map_axis(gx, nx, perx, sx0));
-    const float osign = sx0 * sy0;
+    const uint32_t ocol = (uint32_t)(map_axis(gy, ny, ky, sy0) * nx + map_axis(gx, nx, kx, sx0));
 
     // ring cells this thread loads (at most 2 of the (BX+4)(BY+4) region)
     constexpr int NREG = LX * LY;
     constexpr int PER = (NREG + NT - 1) / NT;
     uint32_t rcol[PER];
-    float rsg[PER];
+    float rsx[PER], rsy[PER];
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
         const int idx = tid + r * NT;
         rcol[r] = 0;
-        rsg[r] = 1.f;
+        rsx[r] = rsy[r] = 1.f;
         if (idx < NREG) {
             const int lx = idx % LX, ly = idx / LX;
-            float s1 = 1.f, s2 = 1.f;
-            rcol[r] = (uint32_t)(map_axis(gy0 - 2 + ly, ny, pery, s2) * nx + map_axis(gx0 - 2 + lx, nx, perx, s1));
-            rsg[r] = s1 * s2;
+            rcol[r] = (uint32_t)(map_axis(gy0 - 2 + ly, ny, ky, rsy[r]) * nx + map_axis(gx0 - 2 + lx, nx, kx, rsx[r]));
         }
     }
     const bool has_d = (EPI == EPI_F32C) && D != nullptr;
     auto zoff = [&](int z, float &s) -> size_t {
         s = 1.f;
-        return (size_t)map_axis(z, nz, perz, s) * plane;
+        return (size_t)map_axis(z, nz, kz, s) * plane;
     };
     auto slot = [](int m) { return ((m % L::NR) + L::NR) % L::NR; };
     // Plane loads keep the raw words; the FP32 increment and the ghost sign
@@ -158,7 +155,7 @@ k_cweno(CwArgs p)
             if (tid + r * NT < NREG) {
                 buf[r].q = load_cell<T, NV>(Q, off + rcol[r]);
                 if (has_d) buf[r].d = load_cell<T, NV>(D, off + rcol[r]);
-                buf[r].s = s * rsg[r];
+                buf[r].s = s;
             }
     };
     auto store_plane = [&](int z, const Raw(&buf)[PER]) {
@@ -172,7 +169,7 @@ k_cweno(CwArgs p)
 #pragma unroll
                     for (int v = 0; v < NV; ++v) c.v[v] += buf[r].d.v[v];
                 }
-                c.v[0] *= T(buf[r].s);
+                apply_bc_sign<T, NV>(c, rsx[r], rsy[r], buf[r].s);
 #pragma unroll
                 for (int v = 0; v < NS; ++v) ring[sl][v][idx / LX][idx % LX] = c.v[v];
             }

@@ -2,34 +2,38 @@
 //
 // Ghost cells are never materialised: a neighbour index outside [0, n) is
 // mapped back into the owned range the way the reference's DMDA ghost fill
-// does it (3d/weno4/plane_wave/hype.c:87-104 periodic wrap;
-// 3d/weno4/plane_wave/rhs_function.c:48-135 odd reflection of component 0
-// for adiabatic walls, applied axis by axis, which makes a corner ghost
-// (-1)^m times the reflected owned value for m reflected axes).
+// does it, axis by axis (which composes the rules at edges and corners):
+//   periodic     wrap (3d/weno4/plane_wave/hype.c:87-104);
+//   adiabatic    mirror, component 0 negated (WAVE2,
+//                3d/weno4/plane_wave/rhs_function.c:48-135);
+//   reflective   mirror, normal velocity component 1+d negated (ACOUSTIC4,
+//                3d/weno3/LinearisedEuler/rhs_function.c:47-176);
+//   transmissive copy of the boundary cell (same file).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace acx {
 
-// Maps a (possibly ghost) index along one axis.  `sign` is multiplied by -1
-// for a reflection.  Indices further than 3 cells outside a non-periodic axis
-// are only reached by threads whose results are discarded; they are clamped
-// so that every load stays inside the array.
-__device__ __forceinline__ int map_axis(int i, int n, int periodic, float &sign)
+// Maps a (possibly ghost) index along one axis of boundary kind `kind`
+// (ACX_PERIODIC 0, ACX_ADIABATIC 1, ACX_REFLECTIVE 2, ACX_TRANSMISSIVE 3).
+// `sign` is multiplied by -1 for a mirror.  Indices further than 3 cells
+// outside a non-periodic axis are only reached by threads whose results are
+// discarded; they are clamped so that every load stays inside the array.
+__device__ __forceinline__ int map_axis(int i, int n, int kind, float &sign)
 {
     if (i >= 0 && i < n) return i;
-    if (periodic) {
+    if (kind == 0) {
         while (i < 0) i += n;
         while (i >= n) i -= n;
         return i;
     }
+    if (kind == 3) return i < 0 ? 0 : n - 1;
+    sign = -sign;
     if (i < 0) {
-        sign = -sign;
         int r = -1 - i;
         return r < n ? r : n - 1;
     }
-    sign = -sign;
     int r = 2 * n - 1 - i;
     return r >= 0 ? r : 0;
 }
@@ -73,6 +77,21 @@ __device__ __forceinline__ Cell<T, NV> load_cell(const T *__restrict__ base, siz
         c.v[0] = a.x; c.v[1] = a.y;
     }
     return c;
+}
+
+// Ghost signs of a mapped cell (sx, sy, sz: -1 where the x, y, z mapping
+// mirrored): WAVE2 (NV = 2) negates component 0 once per mirror; ACOUSTIC4
+// (NV = 4) negates the velocity component normal to each mirror.
+template <typename T, int NV>
+__device__ __forceinline__ void apply_bc_sign(Cell<T, NV> &c, float sx, float sy, float sz)
+{
+    if constexpr (NV == 4) {
+        c.v[1] *= T(sx);
+        c.v[2] *= T(sy);
+        c.v[3] *= T(sz);
+    } else {
+        c.v[0] *= T(sx * sy * sz);
+    }
 }
 
 template <typename T, int NV>

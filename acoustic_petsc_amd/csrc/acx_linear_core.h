@@ -43,15 +43,14 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
 
     // own column
     float sgx = 1.f, sgy = 1.f;
-    const int mx = map_axis(x0 + tx, nx, p.bc[0] == ACX_PERIODIC, sgx);
-    const int my = map_axis(y0 + ty, ny, p.bc[1] == ACX_PERIODIC, sgy);
+    const int mx = map_axis(x0 + tx, nx, p.bc[0], sgx);
+    const int my = map_axis(y0 + ty, ny, p.bc[1], sgy);
     const size_t col = (size_t)my * nx + mx;
-    const float csign = sgx * sgy;
 
     // halo cell owned by this thread (x halo for rows 0..BY-1, y halo for cols 0..BX-1)
     int hx = -1, hy = -1;
     size_t hcol = 0;
-    float hsign = 1.f;
+    float hsx = 1.f, hsy = 1.f;
     if (tid < Tile::NHALO) {
         int t = tid;
         if (t < 6 * BY) {            // x halo: 3 left, 3 right per row
@@ -64,16 +63,14 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
             hx = colx + 3;
             hy = k < 3 ? k : BY + k;
         }
-        float s1 = 1.f, s2 = 1.f;
-        int gx = map_axis(x0 + hx - 3, nx, p.bc[0] == ACX_PERIODIC, s1);
-        int gy = map_axis(y0 + hy - 3, ny, p.bc[1] == ACX_PERIODIC, s2);
+        int gx = map_axis(x0 + hx - 3, nx, p.bc[0], hsx);
+        int gy = map_axis(y0 + hy - 3, ny, p.bc[1], hsy);
         hcol = (size_t)gy * nx + gx;
-        hsign = s1 * s2;
     }
 
     auto zmap = [&](int z, float &s) -> size_t {
         s = 1.f;
-        return (size_t)map_axis(z, nz, p.bc[2] == ACX_PERIODIC, s) * plane;
+        return (size_t)map_axis(z, nz, p.bc[2], s) * plane;
     };
 
     // Lead-plane march.  At iteration j the plane j is staged in LDS (own
@@ -154,7 +151,7 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
 #pragma unroll
                 for (int v = 0; v < NV; ++v) cur.v[v] += S.d.v[v];
             }
-            cur.v[0] *= T(S.zs * csign);
+            apply_bc_sign<T, NV>(cur, sgx, sgy, S.zs);
 
             T nxy[NP];
 #pragma unroll
@@ -168,7 +165,7 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
 #pragma unroll
                         for (int v = 0; v < NV; ++v) hcur.v[v] += S.hd.v[v];
                     }
-                    hcur.v[0] *= T(S.zs * hsign);
+                    apply_bc_sign<T, NV>(hcur, hsx, hsy, S.zs);
 #pragma unroll
                     for (int v = 0; v < NS; ++v) lds[buf][v][hy][hx] = hcur.v[v];
                 }
@@ -309,14 +306,13 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
     const bool has_d = (EPI == EPI_F32C) && D != nullptr;
 
     float sgx = 1.f, sgy = 1.f;
-    const int mx = map_axis(x0 + tx, nx, p.bc[0] == ACX_PERIODIC, sgx);
-    const int my = map_axis(y0 + ty, ny, p.bc[1] == ACX_PERIODIC, sgy);
+    const int mx = map_axis(x0 + tx, nx, p.bc[0], sgx);
+    const int my = map_axis(y0 + ty, ny, p.bc[1], sgy);
     const size_t col = (size_t)my * nx + mx;
-    const float csign = sgx * sgy;
 
     int hx = -1, hy = -1;
     size_t hcol = 0;
-    float hsign = 1.f;
+    float hsx = 1.f, hsy = 1.f;
     if (tid < Tile::NHALO) {
         int t = tid;
         if (t < 6 * BY) {
@@ -329,15 +325,13 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
             hx = colx + 3;
             hy = k < 3 ? k : BY + k;
         }
-        float s1 = 1.f, s2 = 1.f;
-        int gx = map_axis(x0 + hx - 3, nx, p.bc[0] == ACX_PERIODIC, s1);
-        int gy = map_axis(y0 + hy - 3, ny, p.bc[1] == ACX_PERIODIC, s2);
+        int gx = map_axis(x0 + hx - 3, nx, p.bc[0], hsx);
+        int gy = map_axis(y0 + hy - 3, ny, p.bc[1], hsy);
         hcol = (size_t)gy * nx + gx;
-        hsign = s1 * s2;
     }
     auto zmap = [&](int z, float &s) -> size_t {
         s = 1.f;
-        return (size_t)map_axis(z, nz, p.bc[2] == ACX_PERIODIC, s) * plane;
+        return (size_t)map_axis(z, nz, p.bc[2], s) * plane;
     };
     struct Raw {
         Cell<T, NV> q, d;
@@ -348,13 +342,13 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
         r.q = load_cell<T, NV>(Q, off + c);
         if (has_d) r.d = load_cell<T, NV>(D, off + c);
     };
-    auto apply = [&](const Raw &r, float sign) -> Cell<T, NV> {
+    auto apply = [&](const Raw &r, float sx, float sy) -> Cell<T, NV> {
         Cell<T, NV> c = r.q;
         if (has_d) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) c.v[v] += r.d.v[v];
         }
-        c.v[0] *= T(r.s * sign);
+        apply_bc_sign<T, NV>(c, sx, sy, r.s);
         return c;
     };
 
@@ -363,13 +357,13 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
     for (int d = -3; d <= 3; ++d) {
         Raw r;
         issue(r, col, zb + d);
-        qz[d + 3] = apply(r, csign);
+        qz[d + 3] = apply(r, sgx, sgy);
     }
     Cell<T, NV> hv;
     if (hx >= 0) {
         Raw r;
         issue(r, hcol, zb);
-        hv = apply(r, hsign);
+        hv = apply(r, hsx, hsy);
     }
 
     const T inv48h = T(p.inv48h), Kc = T(p.K), irho = T(p.invrho), sLLF = T(p.s);
@@ -480,8 +474,8 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
 
 #pragma unroll
         for (int k = 0; k < 6; ++k) qz[k] = qz[k + 1];
-        qz[6] = apply(qn, csign);
-        if (hx >= 0) hv = apply(hn, hsign);
+        qz[6] = apply(qn, sgx, sgy);
+        if (hx >= 0) hv = apply(hn, hsx, hsy);
         buf ^= 1;
     }
 }

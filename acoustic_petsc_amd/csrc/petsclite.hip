@@ -914,9 +914,13 @@ static PetscErrorCode grid_from_ctx(DM da, const AppCtx *c, acx_grid *g)
                                         {c->back_boundary, c->front_boundary}};
     for (int d = 0; d < 3; ++d) {
         const bool per = da->b[d] == DM_BOUNDARY_PERIODIC;
-        if (!per && (lohi[d][0] != adiabatic || lohi[d][1] != adiabatic))
-            return PetscLiteError(56, "RHSFunction: non-periodic axis %d needs adiabatic walls on both sides", d);
-        g->bc[d] = per ? ACX_PERIODIC : ACX_ADIABATIC;
+        if (per) {
+            g->bc[d] = ACX_PERIODIC;
+            continue;
+        }
+        if (lohi[d][0] != lohi[d][1] || lohi[d][0] == periodic)
+            return PetscLiteError(56, "RHSFunction: non-periodic axis %d needs the same wall kind on both sides", d);
+        g->bc[d] = lohi[d][0] == adiabatic ? ACX_ADIABATIC : lohi[d][0] == reflective ? ACX_REFLECTIVE : ACX_TRANSMISSIVE;
     }
     g->h = c->h;
     g->gamma_hi = c->gammaHi;

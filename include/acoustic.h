@@ -28,8 +28,11 @@ typedef struct {
     PetscReal comp[nVar];
 } Field;
 
-/* boundary kinds of the reference (hype.h:115; 2D: adiabatic_wall) */
-enum bndry_type { periodic, adiabatic };
+/* boundary kinds of the reference programs: periodic, adiabatic (WAVE2,
+ * 3d/weno4/plane_wave/hype.h:115; 2D: adiabatic_wall), reflective and
+ * transmissive (ACOUSTIC4, 3d/weno3/LinearisedEuler/hype.h:111).  Both faces
+ * of a non-periodic axis take the same kind here. */
+enum bndry_type { periodic, adiabatic, reflective, transmissive };
 
 /* system selector: which of the reference's operators RHSFunction applies */
 enum acx_system_type { WAVE_SYSTEM = 0, ACOUSTIC_SYSTEM = 1 };

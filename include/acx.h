@@ -33,13 +33,17 @@ extern "C" {
 #define ACX_WAVE2      0   /* (u, v): u_t = v, v_t = div(c^2 grad u); 3d/weno4/plane_wave */
 #define ACX_ACOUSTIC4  1   /* (p, u, v, w) linearised acoustics, LLF flux; 3d/weno3/LinearisedEuler */
 
-#define ACX_PERIODIC   0   /* DM_BOUNDARY_PERIODIC */
-#define ACX_ADIABATIC  1   /* DM_BOUNDARY_GHOSTED + odd reflection of component 0 */
+#define ACX_PERIODIC      0   /* DM_BOUNDARY_PERIODIC */
+#define ACX_ADIABATIC     1   /* WAVE2: DM_BOUNDARY_GHOSTED + odd reflection of component 0
+                                 (3d/weno4/plane_wave/rhs_function.c:48-135) */
+#define ACX_REFLECTIVE    2   /* ACOUSTIC4: mirror, normal velocity negated
+                                 (3d/weno3/LinearisedEuler/rhs_function.c:47-176) */
+#define ACX_TRANSMISSIVE  3   /* ACOUSTIC4: copy of the boundary cell (same file) */
 
 typedef struct acx_grid {
     int32_t n[3];        /* cells along x, y, z                                */
     int32_t system;      /* ACX_WAVE2 or ACX_ACOUSTIC4                         */
-    int32_t bc[3];       /* ACX_PERIODIC / ACX_ADIABATIC per axis              */
+    int32_t bc[3];       /* boundary kind per axis (both faces of the axis)   */
     int32_t pad_;
     double  h;           /* uniform cell size                                  */
     double  gamma_hi;    /* CWENO linear weight of the 4th-order polynomial;

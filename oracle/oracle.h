@@ -23,7 +23,12 @@ extern "C" {
 /* ---- operators -------------------------------------------------------- */
 
 enum { ORC_WAVE2 = 0, ORC_ACOUSTIC4 = 1 };      /* system switch            */
-enum { ORC_PERIODIC = 0, ORC_ADIABATIC = 1 };   /* per-axis boundary kind   */
+/* per-axis boundary kind: ADIABATIC = odd reflection of component 0 (WAVE2,
+ * 3d/weno4/plane_wave/rhs_function.c:48-135); REFLECTIVE / TRANSMISSIVE =
+ * the LinearisedEuler walls (3d/weno3/LinearisedEuler/rhs_function.c:47-176,
+ * ACOUSTIC4): mirror with the normal velocity negated / copy of the
+ * boundary cell. */
+enum { ORC_PERIODIC = 0, ORC_ADIABATIC = 1, ORC_REFLECTIVE = 2, ORC_TRANSMISSIVE = 3 };
 
 typedef struct {
     int    N[3];        /* cells per axis (x, y, z)                          */
@@ -34,7 +39,7 @@ typedef struct {
     double c0;          /* wave speed                                        */
     double rho0;        /* base density (ACOUSTIC4)                          */
     double cfl;         /* CFL number used for the dt estimate (ACOUSTIC4)   */
-    int    bc[3];       /* ORC_PERIODIC / ORC_ADIABATIC per axis (WAVE2 only) */
+    int    bc[3];       /* boundary kind per axis (both faces)               */
 } orc_grid3d;
 
 int orc_nvar(int system);

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 WAVE2, ACOUSTIC4 = 0, 1
-PERIODIC, ADIABATIC = 0, 1
+PERIODIC, ADIABATIC, REFLECTIVE, TRANSMISSIVE = 0, 1, 2, 3
 
 dp = C.POINTER(C.c_double)
 

@@ -70,6 +70,31 @@ static double *ghost_fill(const orc_grid3d *g, const double *U, int nv)
                     if (k >= Nz) AT(i, j, k) = -AT(i, j, 2 * (Nz - 1) - k + 1);
                 }
 #undef AT
+    /* LinearisedEuler walls (rhs_function.c:47-176): x pass, then y, then z,
+     * each over the whole ghost box; all components copied from the mirror
+     * cell (reflective: irhs = -1-i low, 2(N-1)-i+1 high, normal velocity
+     * component 1+d negated) or from the boundary cell (transmissive). */
+#define CELL(i, j, k) (L + ((((size_t)((k) + GW)) * Gy + ((j) + GW)) * Gx + ((i) + GW)) * nv)
+    const int Nn[3] = {Nx, Ny, Nz};
+    for (int d = 0; d < 3; ++d) {
+        const int kind = g->bc[d];
+        if (kind != ORC_REFLECTIVE && kind != ORC_TRANSMISSIVE) continue;
+        for (int k = -GW; k < Nz + GW; ++k)
+            for (int j = -GW; j < Ny + GW; ++j)
+                for (int i = -GW; i < Nx + GW; ++i) {
+                    const int q = d == 0 ? i : (d == 1 ? j : k), n = Nn[d];
+                    if (q >= 0 && q < n) continue;
+                    int r;
+                    if (kind == ORC_TRANSMISSIVE) r = q < 0 ? 0 : n - 1;
+                    else r = q < 0 ? -1 - q : 2 * (n - 1) - q + 1;
+                    const int si = d == 0 ? r : i, sj = d == 1 ? r : j, sk = d == 2 ? r : k;
+                    const double *src = CELL(si, sj, sk);
+                    double *dst = CELL(i, j, k);
+                    for (int c = 0; c < nv; ++c) dst[c] = src[c];
+                    if (kind == ORC_REFLECTIVE && nv == 4) dst[1 + d] = -dst[1 + d];
+                }
+    }
+#undef CELL
     return L;
 }
 

@@ -55,9 +55,15 @@ def test_invalid_arguments_are_reported(acx):
     g = acx.Grid(n=(2, 16, 16), h=0.1).c_struct()
     rc = lib.acx_rhs_f64(C.byref(g), C.c_void_p(16), C.c_void_p(32), None)
     assert rc == 1 and b"at least 4 cells" in lib.acx_last_error()
-    g = acx.Grid(n=(16, 16, 16), h=0.1, bc=(1, 0, 0)).c_struct()
+    g = acx.Grid(n=(16, 16, 16), h=0.1, bc=(acx.ADIABATIC, 0, 0)).c_struct()   # ACOUSTIC4 default
     rc = lib.acx_rhs_f64(C.byref(g), C.c_void_p(16), C.c_void_p(32), None)
-    assert rc == 2 and b"periodic" in lib.acx_last_error()
+    assert rc == 2 and b"use reflective" in lib.acx_last_error()
+    g = acx.Grid(n=(16, 16, 16), h=0.1, system=acx.WAVE2, bc=(0, acx.TRANSMISSIVE, 0)).c_struct()
+    rc = lib.acx_rhs_f64(C.byref(g), C.c_void_p(16), C.c_void_p(32), None)
+    assert rc == 2 and b"ACOUSTIC4 rules" in lib.acx_last_error()
+    g = acx.Grid(n=(16, 16, 16), h=0.1, bc=(0, 0, 7)).c_struct()
+    rc = lib.acx_rhs_f64(C.byref(g), C.c_void_p(16), C.c_void_p(32), None)
+    assert rc == 1 and b"not a boundary kind" in lib.acx_last_error()
     g = acx.Grid(n=(16, 16, 16), h=0.1).c_struct()
     rc = lib.acx_stage_f64(C.byref(g), C.c_void_p(16), None, C.c_void_p(16), 0.0, 1.0, 1.0, None)
     assert rc == 1

@@ -56,6 +56,12 @@ CASES = [
     ("w2-lin-mixed", (70, 12, 10), 0, (1.0, 0.0), (0, 1, 1), "rough"),
     ("w2-nl-per", (24, 20, 16), 0, (0.85, 0.85), (0, 0, 0), "rough"),
     ("w2-nl-adia", (24, 20, 16), 0, (0.85, 0.85), (1, 1, 1), "rough"),
+    # LinearisedEuler walls: 2 = reflective, 3 = transmissive
+    ("ac4-lin-refl", (24, 20, 16), 1, (1.0, 0.0), (2, 2, 2), "rough"),
+    ("ac4-lin-trans", (24, 20, 16), 1, (1.0, 0.0), (3, 3, 3), "rough"),
+    ("ac4-lin-mixed", (70, 12, 10), 1, (1.0, 0.0), (2, 0, 3), "smooth"),
+    ("ac4-nl-refl", (24, 20, 16), 1, (0.85, 0.85), (2, 2, 2), "rough"),
+    ("ac4-nl-mixed", (20, 24, 14), 1, (0.85, 0.85), (3, 2, 0), "step"),
 ]
 
 
@@ -76,7 +82,7 @@ def test_rhs_parity(name, n, system, gamma, bc, kind, acx, orc, torch_gpu):
     assert err <= TOL, f"{name}: rel L2 {err:.3e}"
 
 
-@pytest.mark.parametrize("name,n,system,gamma,bc,kind", [CASES[0], CASES[1], CASES[3], CASES[7]],
+@pytest.mark.parametrize("name,n,system,gamma,bc,kind", [CASES[0], CASES[1], CASES[3], CASES[7], CASES[13]],
                          ids=["ac4-lin-smooth", "ac4-lin-rough", "ac4-nl-rough", "w2-lin-adia"])
 def test_forced_general_kernel_matches(name, n, system, gamma, bc, kind, acx, orc, torch_gpu, monkeypatch):
     """The general CWENO kernel run with the linear weights must agree with the oracle too."""
@@ -111,6 +117,33 @@ def test_rks3_step_parity(gamma, acx, orc, torch_gpu):
     err = rel_l2(got, ref)
     print(f"[case] rks3 gamma={gamma} rel_l2={err:.3e}", flush=True)
     assert err <= TOL
+
+
+@pytest.mark.parametrize("bc", [(2, 3, 0), (3, 0, 2)])
+def test_walls_rks3_fp64_and_f32c(bc, acx, orc, torch_gpu):
+    """Reflective / transmissive walls through full FP64 steps (oracle, 1e-12)
+    and through the FP32 compensated stepper (FP32 accuracy)."""
+    torch = torch_gpu
+    n = (26, 20, 18)
+    grid = acx.Grid(n=n, h=1.0 / 26, system=acx.ACOUSTIC4, bc=bc)
+    U = make_field(acx, grid, 9, "smooth") * 1e-3
+    dt = 0.4 * grid.h / 3.0
+    print(f"[case] walls {bc}", flush=True)
+    ref = U.copy()
+    g = orc_grid(orc, grid)
+    for _ in range(3):
+        orc.ts_solve("orc_rhs3d_cb_faithful", g, ref, dt, dt, 0, 4)
+    Ud = torch.from_numpy(U).cuda()
+    for _ in range(3):
+        acx.ssp_rks3_step(grid, Ud, dt)
+    err = rel_l2(Ud.cpu().numpy(), ref)
+    st = acx.CompensatedState(grid, torch.from_numpy(U).cuda())
+    for _ in range(3):
+        st.step(dt)
+    err32 = rel_l2(st.value64().cpu().numpy(), ref)
+    print(f"[case] walls {bc}: fp64 {err:.2e} f32c {err32:.2e}", flush=True)
+    assert err <= TOL
+    assert err32 < 1e-6
 
 
 def test_stage_epilogues(acx, orc, torch_gpu):

@@ -304,3 +304,47 @@ def test_acoustic_operator_converges_at_fourth_order(orc):
         errs.append(np.linalg.norm(U - _acoustic_plane_wave(N, k, A, T)) / np.linalg.norm(U))
     rate = math.log2(errs[0] / errs[1])
     assert 3.6 < rate < 4.6, (errs, rate)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("gamma", [(1.0, 0.0), (0.85, 0.85)])
+def test_reflective_wall_is_a_mirrored_periodic_domain(orc, axis, gamma):
+    """LinearisedEuler reflective walls (rhs_function.c:47-176): the RHS on
+    [0, N) with reflective walls along `axis` equals the RHS of the periodic
+    domain [0, 2N) holding the mirror image (pressure and tangential velocity
+    even, normal velocity odd) -- the wrap of the doubled domain reproduces
+    the ghost rule irhs = -1-i / 2N-1-i and the sign of component 1+axis."""
+    rng = np.random.default_rng(30 + axis)
+    n = [10, 9, 8]
+    U = rng.standard_normal((n[2], n[1], n[0], 4)) * 1e-3
+    ax = 2 - axis                      # numpy axis of the grid axis
+    mirror = np.flip(U, axis=ax).copy()
+    mirror[..., 1 + axis] *= -1
+    D = np.concatenate([U, mirror], axis=ax)
+    nd = list(n)
+    nd[axis] *= 2
+    bc = [orc.PERIODIC] * 3
+    bc[axis] = orc.REFLECTIVE
+    g = orc.grid3d(tuple(n), 0.1, orc.ACOUSTIC4, gamma, bc=tuple(bc))
+    gd = orc.grid3d(tuple(nd), 0.1, orc.ACOUSTIC4, gamma)
+    R = orc.rhs3d(g, U)
+    Rd = orc.rhs3d(gd, D)
+    Rd = np.take(Rd, range(n[axis]), axis=ax)
+    assert np.max(np.abs(R - Rd)) <= 1e-14 * np.max(np.abs(Rd))
+
+
+@pytest.mark.parametrize("gamma", [(1.0, 0.0), (0.85, 0.85)])
+def test_transmissive_walls(orc, gamma):
+    """Transmissive walls copy the boundary cell: a state constant along x has
+    no x-flux difference, so its RHS equals the periodic-x RHS; a constant
+    state has zero RHS."""
+    rng = np.random.default_rng(41)
+    n = (9, 10, 8)
+    col = rng.standard_normal((n[2], n[1], 1, 4)) * 1e-3
+    U = np.ascontiguousarray(np.broadcast_to(col, (n[2], n[1], n[0], 4)))
+    gt = orc.grid3d(n, 0.1, orc.ACOUSTIC4, gamma, bc=(orc.TRANSMISSIVE, 0, 0))
+    gp = orc.grid3d(n, 0.1, orc.ACOUSTIC4, gamma)
+    assert np.max(np.abs(orc.rhs3d(gt, U) - orc.rhs3d(gp, U))) <= 1e-14 * np.max(np.abs(orc.rhs3d(gp, U)))
+    gall = orc.grid3d(n, 0.1, orc.ACOUSTIC4, gamma, bc=(orc.TRANSMISSIVE,) * 3)
+    C0 = np.full((n[2], n[1], n[0], 4), 2.5e-4)
+    assert np.max(np.abs(orc.rhs3d(gall, C0))) < 1e-15
