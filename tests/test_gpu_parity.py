@@ -83,7 +83,7 @@ def test_rhs_parity(name, n, system, gamma, bc, kind, acx, orc, torch_gpu):
 
 
 @pytest.mark.parametrize("name,n,system,gamma,bc,kind", [CASES[0], CASES[1], CASES[3], CASES[7], CASES[13]],
-                         ids=["ac4-lin-smooth", "ac4-lin-rough", "ac4-nl-rough", "w2-lin-adia"])
+                         ids=["ac4-lin-smooth", "ac4-lin-rough", "ac4-nl-rough", "w2-lin-adia", "ac4-lin-mixed"])
 def test_forced_general_kernel_matches(name, n, system, gamma, bc, kind, acx, orc, torch_gpu, monkeypatch):
     """The general CWENO kernel run with the linear weights must agree with the oracle too."""
     torch = torch_gpu
