@@ -62,6 +62,10 @@ CASES = [
     ("ac4-lin-mixed", (70, 12, 10), 1, (1.0, 0.0), (2, 0, 3), "smooth"),
     ("ac4-nl-refl", (24, 20, 16), 1, (0.85, 0.85), (2, 2, 2), "rough"),
     ("ac4-nl-mixed", (20, 24, 14), 1, (0.85, 0.85), (3, 2, 0), "step"),
+    # smallest grids the ABI accepts (4 cells per axis): stencils wrap / mirror more than once
+    ("ac4-lin-tiny", (4, 5, 6), 1, (1.0, 0.0), (0, 2, 3), "rough"),
+    ("ac4-nl-tiny", (5, 4, 4), 1, (0.85, 0.85), (0, 0, 2), "rough"),
+    ("w2-nl-tiny", (4, 4, 5), 0, (0.85, 0.85), (1, 0, 1), "rough"),
 ]
 
 
