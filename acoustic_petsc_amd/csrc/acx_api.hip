@@ -105,14 +105,13 @@ enum { EPI_RHS = 0, EPI_AXPY = 1, EPI_COMB = 2, EPI_F32C = 3 };
 
 // One FP64 stage: OUT = a*U0 + b*Q + c*F(Q), epilogue chosen from (a, b).
 static int stage64(const acx_grid *g, const double *Q, const double *U0, double *OUT,
-                   double a, double b, double c, int epi, hipStream_t s, int zrev = 0)
+                   double a, double b, double c, int epi, hipStream_t s)
 {
     if (is_linear(g)) {
         LinArgs la;
         fill_lin(la, g);
         la.q = Q; la.u0 = U0; la.out = OUT;
         la.a = a; la.b = b; la.c = c;
-        la.zrev = zrev;
         return launch_linear(la, g->system, epi, false, s);
     }
     CwArgs ca;
@@ -189,14 +188,10 @@ int acx_ssp_rks3_step_f64(const acx_grid *g, double *U, double *W0, double *W1, 
     if (!U || !W0 || !W1 || U == W0 || U == W1 || W0 == W1) return set_error(ACX_EINVAL, "U, W0, W1 must be distinct");
     hipStream_t s = (hipStream_t)stream;
     // TSSSPRKS3, s = 4 (n = 2, r = 2): stage times 0, 1/2, 1, 1/2.
-    // Stages 2 and 4 walk the z-chunks from the top down: each stage then
-    // starts on the planes its predecessor wrote last, which are still in the
-    // memory-side cache.
-    const int alt = env_int("ACX_ZALT", 1) ? 1 : 0;
     if ((rc = stage64(g, U, nullptr, W0, 0.0, 1.0, dt / 2, EPI_AXPY, s))) return rc;
-    if ((rc = stage64(g, W0, nullptr, W1, 0.0, 1.0, dt / 2, EPI_AXPY, s, alt))) return rc;
+    if ((rc = stage64(g, W0, nullptr, W1, 0.0, 1.0, dt / 2, EPI_AXPY, s))) return rc;
     if ((rc = stage64(g, W1, U, W0, 1. * 2 / 3, (2 - 1.) * 1. / 3, (2 - 1.) * dt / (2 * 3), EPI_COMB, s))) return rc;
-    return stage64(g, W0, nullptr, U, 0.0, 1.0, dt / 2, EPI_AXPY, s, alt);
+    return stage64(g, W0, nullptr, U, 0.0, 1.0, dt / 2, EPI_AXPY, s);
 }
 
 int acx_ssp_rks3_step_f32c(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,

@@ -20,7 +20,6 @@ struct LinArgs {
     int ntx, nty, ntz;   // tile counts (filled by the launcher)
     int bc[3];
     int final_stage;
-    int zrev;          // 1: z-chunks handed out from the top of the grid down
     double a, b, c;    // epilogue coefficients
     double K, invrho, s, inv48h, lapc;
 };

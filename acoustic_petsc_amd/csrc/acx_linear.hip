@@ -29,9 +29,7 @@ k_linear(LinArgs p)
     using Tile = LinTile<T, SYS, BX, BY>;
     __shared__ T lds[2][Tile::NS][BY + 6][BX + 6];
     const int tile = xcd_logical_tile(blockIdx.x, gridDim.x);
-    int tz = tile / (p.ntx * p.nty);
-    if (p.zrev) tz = p.ntz - 1 - tz;
-    const int x0 = (tile % p.ntx) * BX, y0 = ((tile / p.ntx) % p.nty) * BY, zb = tz * p.zc;
+    const int x0 = (tile % p.ntx) * BX, y0 = ((tile / p.ntx) % p.nty) * BY, zb = (tile / (p.ntx * p.nty)) * p.zc;
     if constexpr (PF <= 0)
         linear_tile_centre<T, SYS, BX, BY, EPI, (PF < 0)>(p, x0, y0, zb, lds);
     else
