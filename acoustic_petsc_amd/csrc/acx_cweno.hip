@@ -235,37 +235,64 @@ k_cweno(CwArgs p)
             const T(*P0)[LX] = ring[s0][v];
             const T(*P1)[LX] = ring[s1][v];
             const T(*P2)[LX] = ring[s2][v];
-            T st[33];
-            const T zm2 = ring[sm2][v][ly][lx], zp2 = ring[sp2][v][ly][lx];
-            st[0] = P1[ly][lx];
-            st[1] = P1[ly][lx - 2]; st[2] = P1[ly][lx - 1]; st[3] = P1[ly][lx + 1]; st[4] = P1[ly][lx + 2];
-            st[5] = P1[ly - 2][lx]; st[6] = P1[ly - 1][lx]; st[7] = P1[ly + 1][lx]; st[8] = P1[ly + 2][lx];
-            st[9] = zm2; st[10] = P0[ly][lx]; st[11] = P2[ly][lx]; st[12] = zp2;
-            st[13] = P1[ly + 1][lx + 1]; st[14] = P1[ly - 1][lx + 1]; st[15] = P1[ly + 1][lx - 1]; st[16] = P1[ly - 1][lx - 1];
-            // yz diagonals (y,z): ++ +- -+ --
-            st[17] = P2[ly + 1][lx]; st[18] = P0[ly + 1][lx]; st[19] = P2[ly - 1][lx]; st[20] = P0[ly - 1][lx];
-            // zx diagonals (z,x): ++ +- -+ --
-            st[21] = P2[ly][lx + 1]; st[22] = P2[ly][lx - 1]; st[23] = P0[ly][lx + 1]; st[24] = P0[ly][lx - 1];
-            // corners (x,y,z): +++ -++ +-+ ++- --+ -+- +-- ---
-            st[25] = P2[ly + 1][lx + 1]; st[26] = P2[ly + 1][lx - 1]; st[27] = P2[ly - 1][lx + 1]; st[28] = P0[ly + 1][lx + 1];
-            st[29] = P2[ly - 1][lx - 1]; st[30] = P0[ly + 1][lx - 1]; st[31] = P0[ly - 1][lx + 1]; st[32] = P0[ly - 1][lx - 1];
+            const T(*M2)[LX] = ring[sm2][v];
+            const T(*Q2)[LX] = ring[sp2][v];
+            // stencil slot q -> LDS (slot order of cweno_weights)
+            auto get = [&](int q) -> T {
+                switch (q) {
+                case 0: return P1[ly][lx];
+                case 1: return P1[ly][lx - 2];
+                case 2: return P1[ly][lx - 1];
+                case 3: return P1[ly][lx + 1];
+                case 4: return P1[ly][lx + 2];
+                case 5: return P1[ly - 2][lx];
+                case 6: return P1[ly - 1][lx];
+                case 7: return P1[ly + 1][lx];
+                case 8: return P1[ly + 2][lx];
+                case 9: return M2[ly][lx];
+                case 10: return P0[ly][lx];
+                case 11: return P2[ly][lx];
+                case 12: return Q2[ly][lx];
+                case 13: return P1[ly + 1][lx + 1];   // xy: ++ +- -+ --
+                case 14: return P1[ly - 1][lx + 1];
+                case 15: return P1[ly + 1][lx - 1];
+                case 16: return P1[ly - 1][lx - 1];
+                case 17: return P2[ly + 1][lx];       // yz (y,z): ++ +- -+ --
+                case 18: return P0[ly + 1][lx];
+                case 19: return P2[ly - 1][lx];
+                case 20: return P0[ly - 1][lx];
+                case 21: return P2[ly][lx + 1];       // zx (z,x): ++ +- -+ --
+                case 22: return P2[ly][lx - 1];
+                case 23: return P0[ly][lx + 1];
+                case 24: return P0[ly][lx - 1];
+                case 25: return P2[ly + 1][lx + 1];   // corners (x,y,z): +++ -++ +-+ ++- --+ -+- +-- ---
+                case 26: return P2[ly + 1][lx - 1];
+                case 27: return P2[ly - 1][lx + 1];
+                case 28: return P0[ly + 1][lx + 1];
+                case 29: return P2[ly - 1][lx - 1];
+                case 30: return P0[ly + 1][lx - 1];
+                case 31: return P0[ly - 1][lx + 1];
+                default: return P0[ly - 1][lx - 1];
+                }
+            };
 
             Axis3<T> ax;
             bool full = true;
             if constexpr (SEL) full = (fl >> v) & 1;
             if (full) {
-                cweno_axes<T>(st, p, ax);
+                cweno_axes_lds<T>(get, p, ax);
             } else {
-                axis_linear(st[1], st[2], st[0], st[3], st[4], ax.c1[0], ax.c2[0], ax.c3[0]);
-                axis_linear(st[5], st[6], st[0], st[7], st[8], ax.c1[1], ax.c2[1], ax.c3[1]);
-                axis_linear(st[9], st[10], st[0], st[11], st[12], ax.c1[2], ax.c2[2], ax.c3[2]);
+                axis_linear(get(1), get(2), get(0), get(3), get(4), ax.c1[0], ax.c2[0], ax.c3[0]);
+                axis_linear(get(5), get(6), get(0), get(7), get(8), ax.c1[1], ax.c2[1], ax.c3[1]);
+                axis_linear(get(9), get(10), get(0), get(11), get(12), ax.c1[2], ax.c2[2], ax.c3[2]);
             }
             T m3[3], p3[3];
-            face_q<T, SYS>(st[0], ax, m3, p3);
+            face_q<T, SYS>(get(0), ax, m3, p3);
             fmx[v] = m3[0]; fpx[v] = p3[0];
             s_fym[v][ty][tx] = m3[1];
             fpy[v] = p3[1];
             fmz[v] = m3[2]; fpz[v] = p3[2];
+            asm volatile("" ::: "memory");   // one component's stencil live at a time
         }
         __syncthreads();   // B1: -y states visible; every read of plane m-1 is done
 

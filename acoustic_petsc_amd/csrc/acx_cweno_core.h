@@ -40,75 +40,90 @@ __device__ __forceinline__ void axis_linear(T m2, T m1, T u0, T p1, T p2, T &c1,
     c3 = T(1.0 / 12.0) * (-m2 + T(2) * (m1 - p1) + p2);
 }
 
-// Full CWENO4: returns the axis coefficients (c1,c2,c3 per axis) and wt_ratio;
-// wn (if not null) receives the eight normalised weights (R4, R3[0..6]).
-template <typename T>
-__device__ __forceinline__ T cweno_axes(const T (&s)[33], const CwArgs &p, Axis3<T> &o, T *wn = nullptr)
+// Smoothness indicators and normalised nonlinear weights of one cell and
+// component; get(q) returns stencil slot q (0: centre; 1-4, 5-8, 9-12: the
+// x, y, z arms m2 m1 p1 p2; 13-16 xy, 17-20 yz, 21-24 zx diagonals; 25-32
+// corners).  wt = wt_ratio (reconstruction.c:236), w3[7] the R3 weights.
+// The indicators are accumulated in three groups (axis arms, edge diagonals,
+// corners); with LDS-backed values the memory clobbers between the groups keep
+// only one group live at a time.
+template <typename T, typename Get>
+__device__ __forceinline__ void cweno_weights(Get get, const CwArgs &p, T &wt, T (&w3)[7])
 {
-    // stencil slots (see gather order in the kernel)
-    const T u0 = s[0];
-    const T xm2 = s[1], xm1 = s[2], xp1 = s[3], xp2 = s[4];
-    const T ym2 = s[5], ym1 = s[6], yp1 = s[7], yp2 = s[8];
-    const T zm2 = s[9], zm1 = s[10], zp1 = s[11], zp2 = s[12];
-    const T xy_pp = s[13], xy_pm = s[14], xy_mp = s[15], xy_mm = s[16];
-    const T yz_pp = s[17], yz_pm = s[18], yz_mp = s[19], yz_mm = s[20];
-    const T zx_pp = s[21], zx_pm = s[22], zx_mp = s[23], zx_mm = s[24];
-    const T ppp = s[25], mpp = s[26], pmp = s[27], ppm = s[28];
-    const T mmp = s[29], mpm = s[30], pmm = s[31], mmm = s[32];
-
-    // R4 (reconstruction.c:90-108)
-    T ux, uy, uz, uxx, uyy, uzz, uxxx, uyyy, uzzz;
-    axis_linear(xm2, xm1, u0, xp1, xp2, ux, uxx, uxxx);
-    axis_linear(ym2, ym1, u0, yp1, yp2, uy, uyy, uyyy);
-    axis_linear(zm2, zm1, u0, zp1, zp2, uz, uzz, uzzz);
-    const T uxy = T(0.25) * (xy_mm - xy_mp - xy_pm + xy_pp);
-    const T uyz = T(0.25) * (yz_mm - yz_mp - yz_pm + yz_pp);
-    const T uxz = T(0.25) * (zx_mm - zx_mp - zx_pm + zx_pp);
-    const T uxxy = T(0.25) * (-xy_mm + xy_mp - xy_pm + xy_pp) + T(0.5) * (ym1 - yp1);
-    const T uxyy = T(0.5) * (xm1 - xp1) + T(0.25) * (xy_pp - xy_mm - xy_mp + xy_pm);
-    const T uyyz = T(0.25) * (-yz_mm + yz_mp - yz_pm + yz_pp) + T(0.5) * (zm1 - zp1);
-    const T uyzz = T(0.5) * (ym1 - yp1) + T(0.25) * (yz_pp - yz_mm - yz_mp + yz_pm);
-    const T uxxz = T(0.25) * (-zx_mm + zx_pm - zx_mp + zx_pp) + T(0.5) * (zm1 - zp1);
-    const T uxzz = T(0.5) * (xm1 - xp1) + T(0.25) * (zx_pp - zx_mm - zx_pm + zx_mp);
-    const T uxyz = T(0.125) * (ppp - mpp - pmp - ppm + mmp + mpm + pmm - mmm);
-
     const T k13 = T(13.0 / 3.0), k7 = T(7.0 / 6.0);
-    const T is4 = ux * ux + uy * uy + uz * uz + k13 * (uxx * uxx + uyy * uyy + uzz * uzz) +
-                  k7 * (uxy * uxy + uyz * uyz + uxz * uxz) + T(0.2) * (ux * uxxx + uy * uyyy + uz * uzzz) +
-                  T(4.7) * (uxxy * uxxy + uxxz * uxxz + uxyy * uxyy + uxzz * uxzz + uyyz * uyyz + uyzz * uyzz) +
-                  T(39.06) * (uxxx * uxxx + uyyy * uyyy + uzzz * uzzz) + T(61.0 / 48.0) * uxyz * uxyz;
-
-    // R3 stencils (reconstruction.c:121-203): central, +x, -x, +y, -y, +z, -z.
-    // Central derivatives coincide with the R4 ones for xx, yy, zz, xy, yz, xz.
-    const T cx = T(0.5) * (xp1 - xm1), cy = T(0.5) * (yp1 - ym1), cz = T(0.5) * (zp1 - zm1);
-    const T dx1 = T(-1.5) * u0 + T(2) * xp1 - T(0.5) * xp2, dxx1 = T(0.5) * (u0 - T(2) * xp1 + xp2);
-    const T dxy1 = T(0.5) * (xy_pp - xy_pm) - cy, dxz1 = T(0.5) * (zx_pp - zx_mp) - cz;
-    const T dx2 = T(-2) * xm1 + T(0.5) * xm2 + T(1.5) * u0, dxx2 = T(0.5) * (xm2 - T(2) * xm1 + u0);
-    const T dxy2 = T(0.5) * (xy_mm - xy_mp) + cy, dxz2 = T(0.5) * (zx_mm - zx_pm) + cz;
-    const T dy3 = T(-1.5) * u0 + T(2) * yp1 - T(0.5) * yp2, dyy3 = T(0.5) * (u0 - T(2) * yp1 + yp2);
-    const T dxy3 = T(0.5) * (xy_pp - xy_mp) - cx, dyz3 = T(0.5) * (yz_pp - yz_pm) - cz;
-    const T dy4 = T(-2) * ym1 + T(0.5) * ym2 + T(1.5) * u0, dyy4 = T(0.5) * (ym2 - T(2) * ym1 + u0);
-    const T dxy4 = T(0.5) * (xy_mm - xy_pm) + cx, dyz4 = T(0.5) * (yz_mm - yz_mp) + cz;
-    const T dz5 = T(-1.5) * u0 + T(2) * zp1 - T(0.5) * zp2, dzz5 = T(0.5) * (u0 - T(2) * zp1 + zp2);
-    const T dyz5 = T(0.5) * (yz_pp - yz_mp) - cy, dxz5 = T(0.5) * (zx_pp - zx_pm) - cx;
-    const T dz6 = T(-2) * zm1 + T(0.5) * zm2 + T(1.5) * u0, dzz6 = T(0.5) * (zm2 - T(2) * zm1 + u0);
-    const T dyz6 = T(0.5) * (yz_mm - yz_pm) + cy, dxz6 = T(0.5) * (zx_mm - zx_mp) + cx;
-
-    const T Cx = cx * cx + k13 * uxx * uxx, Cy = cy * cy + k13 * uyy * uyy, Cz = cz * cz + k13 * uzz * uzz;
-    const T Cxy = k7 * uxy * uxy, Cyz = k7 * uyz * uyz, Cxz = k7 * uxz * uxz;
-    T is3[7];
-    is3[0] = Cx + Cy + Cz + Cxy + Cyz + Cxz;
-    is3[1] = (dx1 * dx1 + k13 * dxx1 * dxx1) + Cy + Cz + k7 * dxy1 * dxy1 + Cyz + k7 * dxz1 * dxz1;
-    is3[2] = (dx2 * dx2 + k13 * dxx2 * dxx2) + Cy + Cz + k7 * dxy2 * dxy2 + Cyz + k7 * dxz2 * dxz2;
-    is3[3] = Cx + (dy3 * dy3 + k13 * dyy3 * dyy3) + Cz + k7 * dxy3 * dxy3 + k7 * dyz3 * dyz3 + Cxz;
-    is3[4] = Cx + (dy4 * dy4 + k13 * dyy4 * dyy4) + Cz + k7 * dxy4 * dxy4 + k7 * dyz4 * dyz4 + Cxz;
-    is3[5] = Cx + Cy + (dz5 * dz5 + k13 * dzz5 * dzz5) + Cxy + k7 * dyz5 * dyz5 + k7 * dxz5 * dxz5;
-    is3[6] = Cx + Cy + (dz6 * dz6 + k13 * dzz6 * dzz6) + Cxy + k7 * dyz6 * dyz6 + k7 * dxz6 * dxz6;
+    T is4, is3[7], cx, cy, cz;
+    {   // axis arms: R4 x/y/z derivatives and the sided R3 stencils (reconstruction.c:90-101,121-203)
+        const T u0 = get(0);
+        const T xm2 = get(1), xm1 = get(2), xp1 = get(3), xp2 = get(4);
+        const T ym2 = get(5), ym1 = get(6), yp1 = get(7), yp2 = get(8);
+        const T zm2 = get(9), zm1 = get(10), zp1 = get(11), zp2 = get(12);
+        T ux, uy, uz, uxx, uyy, uzz, uxxx, uyyy, uzzz;
+        axis_linear(xm2, xm1, u0, xp1, xp2, ux, uxx, uxxx);
+        axis_linear(ym2, ym1, u0, yp1, yp2, uy, uyy, uyyy);
+        axis_linear(zm2, zm1, u0, zp1, zp2, uz, uzz, uzzz);
+        is4 = ux * ux + uy * uy + uz * uz + k13 * (uxx * uxx + uyy * uyy + uzz * uzz) +
+              T(0.2) * (ux * uxxx + uy * uyyy + uz * uzzz) + T(39.06) * (uxxx * uxxx + uyyy * uyyy + uzzz * uzzz);
+        cx = T(0.5) * (xp1 - xm1);
+        cy = T(0.5) * (yp1 - ym1);
+        cz = T(0.5) * (zp1 - zm1);
+        const T dx1 = T(-1.5) * u0 + T(2) * xp1 - T(0.5) * xp2, dxx1 = T(0.5) * (u0 - T(2) * xp1 + xp2);
+        const T dx2 = T(-2) * xm1 + T(0.5) * xm2 + T(1.5) * u0, dxx2 = T(0.5) * (xm2 - T(2) * xm1 + u0);
+        const T dy3 = T(-1.5) * u0 + T(2) * yp1 - T(0.5) * yp2, dyy3 = T(0.5) * (u0 - T(2) * yp1 + yp2);
+        const T dy4 = T(-2) * ym1 + T(0.5) * ym2 + T(1.5) * u0, dyy4 = T(0.5) * (ym2 - T(2) * ym1 + u0);
+        const T dz5 = T(-1.5) * u0 + T(2) * zp1 - T(0.5) * zp2, dzz5 = T(0.5) * (u0 - T(2) * zp1 + zp2);
+        const T dz6 = T(-2) * zm1 + T(0.5) * zm2 + T(1.5) * u0, dzz6 = T(0.5) * (zm2 - T(2) * zm1 + u0);
+        // central R3 shares the R4 second derivatives
+        const T Cx = cx * cx + k13 * uxx * uxx, Cy = cy * cy + k13 * uyy * uyy, Cz = cz * cz + k13 * uzz * uzz;
+        is3[0] = Cx + Cy + Cz;
+        is3[1] = (dx1 * dx1 + k13 * dxx1 * dxx1) + Cy + Cz;
+        is3[2] = (dx2 * dx2 + k13 * dxx2 * dxx2) + Cy + Cz;
+        is3[3] = Cx + (dy3 * dy3 + k13 * dyy3 * dyy3) + Cz;
+        is3[4] = Cx + (dy4 * dy4 + k13 * dyy4 * dyy4) + Cz;
+        is3[5] = Cx + Cy + (dz5 * dz5 + k13 * dzz5 * dzz5);
+        is3[6] = Cx + Cy + (dz6 * dz6 + k13 * dzz6 * dzz6);
+    }
+    asm volatile("" ::: "memory");
+    {   // edge diagonals: mixed derivatives (reconstruction.c:102-107, sided R3 cross terms)
+        const T ym1 = get(6), yp1 = get(7), zm1 = get(10), zp1 = get(11), xm1 = get(2), xp1 = get(3);
+        const T xy_pp = get(13), xy_pm = get(14), xy_mp = get(15), xy_mm = get(16);
+        const T yz_pp = get(17), yz_pm = get(18), yz_mp = get(19), yz_mm = get(20);
+        const T zx_pp = get(21), zx_pm = get(22), zx_mp = get(23), zx_mm = get(24);
+        const T uxy = T(0.25) * (xy_mm - xy_mp - xy_pm + xy_pp);
+        const T uyz = T(0.25) * (yz_mm - yz_mp - yz_pm + yz_pp);
+        const T uxz = T(0.25) * (zx_mm - zx_mp - zx_pm + zx_pp);
+        const T uxxy = T(0.25) * (-xy_mm + xy_mp - xy_pm + xy_pp) + T(0.5) * (ym1 - yp1);
+        const T uxyy = T(0.5) * (xm1 - xp1) + T(0.25) * (xy_pp - xy_mm - xy_mp + xy_pm);
+        const T uyyz = T(0.25) * (-yz_mm + yz_mp - yz_pm + yz_pp) + T(0.5) * (zm1 - zp1);
+        const T uyzz = T(0.5) * (ym1 - yp1) + T(0.25) * (yz_pp - yz_mm - yz_mp + yz_pm);
+        const T uxxz = T(0.25) * (-zx_mm + zx_pm - zx_mp + zx_pp) + T(0.5) * (zm1 - zp1);
+        const T uxzz = T(0.5) * (xm1 - xp1) + T(0.25) * (zx_pp - zx_mm - zx_pm + zx_mp);
+        is4 += k7 * (uxy * uxy + uyz * uyz + uxz * uxz) +
+               T(4.7) * (uxxy * uxxy + uxxz * uxxz + uxyy * uxyy + uxzz * uxzz + uyyz * uyyz + uyzz * uyzz);
+        const T dxy1 = T(0.5) * (xy_pp - xy_pm) - cy, dxz1 = T(0.5) * (zx_pp - zx_mp) - cz;
+        const T dxy2 = T(0.5) * (xy_mm - xy_mp) + cy, dxz2 = T(0.5) * (zx_mm - zx_pm) + cz;
+        const T dxy3 = T(0.5) * (xy_pp - xy_mp) - cx, dyz3 = T(0.5) * (yz_pp - yz_pm) - cz;
+        const T dxy4 = T(0.5) * (xy_mm - xy_pm) + cx, dyz4 = T(0.5) * (yz_mm - yz_mp) + cz;
+        const T dyz5 = T(0.5) * (yz_pp - yz_mp) - cy, dxz5 = T(0.5) * (zx_pp - zx_pm) - cx;
+        const T dyz6 = T(0.5) * (yz_mm - yz_pm) + cy, dxz6 = T(0.5) * (zx_mm - zx_mp) + cx;
+        const T Cxy = k7 * uxy * uxy, Cyz = k7 * uyz * uyz, Cxz = k7 * uxz * uxz;
+        is3[0] += Cxy + Cyz + Cxz;
+        is3[1] += k7 * dxy1 * dxy1 + Cyz + k7 * dxz1 * dxz1;
+        is3[2] += k7 * dxy2 * dxy2 + Cyz + k7 * dxz2 * dxz2;
+        is3[3] += k7 * dxy3 * dxy3 + k7 * dyz3 * dyz3 + Cxz;
+        is3[4] += k7 * dxy4 * dxy4 + k7 * dyz4 * dyz4 + Cxz;
+        is3[5] += Cxy + k7 * dyz5 * dyz5 + k7 * dxz5 * dxz5;
+        is3[6] += Cxy + k7 * dyz6 * dyz6 + k7 * dxz6 * dxz6;
+    }
+    asm volatile("" ::: "memory");
+    {   // corners: u_xyz (reconstruction.c:108)
+        const T uxyz = T(0.125) * (get(25) - get(26) - get(27) - get(28) + get(29) + get(30) + get(31) - get(32));
+        is4 += T(61.0 / 48.0) * uxyz * uxyz;
+    }
 
     // nonlinear weights, p = 4, eps = 1e-12 (reconstruction.c:118,205-218)
     const T eps = T(1.0e-12);
     const T g4 = T(p.g4), g3c = T(p.g3c), g3s = T(p.g3s);
-    T w4, w3[7], wt;
+    T w4;
     if constexpr (sizeof(T) == 8) {
         w4 = pos_pow4_inv(g4, is4 + eps);
         T total = w4;
@@ -141,16 +156,29 @@ __device__ __forceinline__ T cweno_axes(const T (&s)[33], const CwArgs &p, Axis3
 #pragma unroll
         for (int k = 0; k < 7; ++k) w3[k] *= it;
     }
+}
 
-    if (wn) {
-        wn[0] = wt * T(p.g4);
-#pragma unroll
-        for (int k = 0; k < 7; ++k) wn[1 + k] = w3[k];
-    }
-
-    // CWENO combination, reconstruction.c:223-257, for the face-relevant
-    // coefficients only.  lin = sum gamma_k R3_k, nl = sum w_k R3_k.
-    const T gc = g3c + T(4) * g3s;                // weight of stencils central along an axis
+// CWENO combination (reconstruction.c:223-257) for the face-relevant
+// coefficients only, from the 13 axis values (u0 and the x, y, z arms
+// m2, m1, p1, p2) and the weights.  lin = sum gamma_k R3_k, nl = sum w_k R3_k.
+template <typename T>
+__device__ __forceinline__ void cweno_combine(T u0, T xm2, T xm1, T xp1, T xp2, T ym2, T ym1, T yp1, T yp2, T zm2,
+                                              T zm1, T zp1, T zp2, T wt, const T (&w3)[7], const CwArgs &p,
+                                              Axis3<T> &o)
+{
+    const T g3s = T(p.g3s);
+    const T gc = T(p.g3c) + T(4) * g3s;           // weight of stencils central along an axis
+    T ux, uy, uz, uxx, uyy, uzz, uxxx, uyyy, uzzz;
+    axis_linear(xm2, xm1, u0, xp1, xp2, ux, uxx, uxxx);
+    axis_linear(ym2, ym1, u0, yp1, yp2, uy, uyy, uyyy);
+    axis_linear(zm2, zm1, u0, zp1, zp2, uz, uzz, uzzz);
+    const T cx = T(0.5) * (xp1 - xm1), cy = T(0.5) * (yp1 - ym1), cz = T(0.5) * (zp1 - zm1);
+    const T dx1 = T(-1.5) * u0 + T(2) * xp1 - T(0.5) * xp2, dxx1 = T(0.5) * (u0 - T(2) * xp1 + xp2);
+    const T dx2 = T(-2) * xm1 + T(0.5) * xm2 + T(1.5) * u0, dxx2 = T(0.5) * (xm2 - T(2) * xm1 + u0);
+    const T dy3 = T(-1.5) * u0 + T(2) * yp1 - T(0.5) * yp2, dyy3 = T(0.5) * (u0 - T(2) * yp1 + yp2);
+    const T dy4 = T(-2) * ym1 + T(0.5) * ym2 + T(1.5) * u0, dyy4 = T(0.5) * (ym2 - T(2) * ym1 + u0);
+    const T dz5 = T(-1.5) * u0 + T(2) * zp1 - T(0.5) * zp2, dzz5 = T(0.5) * (u0 - T(2) * zp1 + zp2);
+    const T dz6 = T(-2) * zm1 + T(0.5) * zm2 + T(1.5) * u0, dzz6 = T(0.5) * (zm2 - T(2) * zm1 + u0);
     const T Wx = w3[0] + w3[3] + w3[4] + w3[5] + w3[6];
     const T Wy = w3[0] + w3[1] + w3[2] + w3[5] + w3[6];
     const T Wz = w3[0] + w3[1] + w3[2] + w3[3] + w3[4];
@@ -163,8 +191,37 @@ __device__ __forceinline__ T cweno_axes(const T (&s)[33], const CwArgs &p, Axis3
     o.c1[2] = wt * (uz - (gc * cz + g3s * (dz5 + dz6))) + (Wz * cz + w3[5] * dz5 + w3[6] * dz6);
     o.c2[2] = wt * (uzz - (gc * uzz + g3s * (dzz5 + dzz6))) + (Wz * uzz + w3[5] * dzz5 + w3[6] * dzz6);
     o.c3[2] = wt * uzzz;
+}
+
+// Full CWENO4 from a register array of the 33 stencil values: returns the
+// axis coefficients (c1,c2,c3 per axis) and wt_ratio; wn (if not null)
+// receives the eight normalised weights (R4, R3[0..6]).
+template <typename T>
+__device__ __forceinline__ T cweno_axes(const T (&s)[33], const CwArgs &p, Axis3<T> &o, T *wn = nullptr)
+{
+    T wt, w3[7];
+    cweno_weights<T>([&](int q) { return s[q]; }, p, wt, w3);
+    if (wn) {
+        wn[0] = wt * T(p.g4);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) wn[1 + k] = w3[k];
+    }
+    cweno_combine<T>(s[0], s[1], s[2], s[3], s[4], s[5], s[6], s[7], s[8], s[9], s[10], s[11], s[12], wt, w3, p, o);
     return wt;
 }
 
+// Full CWENO4 from values held in LDS, get(q) returning stencil slot q: the
+// weights are formed group by group, then the 13 axis values are read again
+// for the combination (the memory clobbers keep the compiler from holding all
+// 33 values live across the weight arithmetic).
+template <typename T, typename Get>
+__device__ __forceinline__ void cweno_axes_lds(Get get, const CwArgs &p, Axis3<T> &o)
+{
+    T wt, w3[7];
+    cweno_weights<T>(get, p, wt, w3);
+    asm volatile("" ::: "memory");
+    cweno_combine<T>(get(0), get(1), get(2), get(3), get(4), get(5), get(6), get(7), get(8), get(9), get(10),
+                     get(11), get(12), wt, w3, p, o);
+}
 
 } // namespace acx

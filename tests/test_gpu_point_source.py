@@ -73,9 +73,9 @@ def test_green_function_and_ctcs_trace(tmp_path):
     ip, ie = int(np.argmax(u)), int(np.argmax(exact))
     print(f"[case] Green: rel L2 {rel:.3f}, peak {u[ip]:.4e} at {t[ip]:.4f} s vs {exact[ie]:.4e} at {t[ie]:.4f} s",
           flush=True)
-    assert rel < 0.25
-    assert abs(u[ip] - exact[ie]) < 0.1 * exact[ie]
-    assert abs(ip - ie) <= 3
+    assert rel < 0.02
+    assert abs(u[ip] - exact[ie]) < 0.01 * exact[ie]
+    assert ip == ie
     with open(os.path.join(ROOT, "tests", "golden", "ctcs_point_source_p.json")) as f:
         g = json.load(f)
     key_t = next(k for k in g if k.startswith("t"))
@@ -84,4 +84,4 @@ def test_green_function_and_ctcs_trace(tmp_path):
     uc = np.interp(tc, t, u)
     relc = np.linalg.norm(uc - pc) / np.linalg.norm(pc)
     print(f"[case] vs CTCS p.dat: rel L2 {relc:.3f}", flush=True)
-    assert relc < 0.25
+    assert relc < 0.02
