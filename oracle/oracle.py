@@ -75,6 +75,8 @@ def lib():
         L.orc_standing_wave2d_init.argtypes = [C.c_int, dp]
         L.orc_standing_wave2d_error.argtypes = [C.c_int, C.c_double, dp, dp, dp]
         L.orc_ctcs3d_run.argtypes = [C.POINTER(CT), C.c_int, dp]
+        L.orc_ctcs3d_update.argtypes = [C.POINTER(CT), dp, dp, dp, dp, C.c_int]
+        L.orc_ctcs3d_update.restype = C.c_double
         L.orc_rhs3d_cb_point_source.argtypes = [C.c_void_p, C.c_double, dp, dp]
         L.orc_weno_source.argtypes = [C.c_double, C.c_double]
         L.orc_weno_source.restype = C.c_double
@@ -150,6 +152,20 @@ def ctcs_trace(n=(100, 100, 100), dx=0.5, dt=0.0005, c0=250.0, stencil=2, src=(5
     tr = np.zeros(nt)
     lib().orc_ctcs3d_run(C.byref(c), nt, P(tr))
     return tr
+
+
+def ctcs_steps(n, d, dt, stencil, src, rcv, Pold, P0, Cf, nt, it0=0, f0=100.0):
+    """nt leapfrog updates (wave.c:109-121) from the pair (Pold, P0), per-cell speed
+    Cf (numpy (nz,ny,nx)); returns (Pold, P, receiver trace)."""
+    c = CT(tuple(n), d[0], d[1], d[2], dt, 0.0, stencil, tuple(src), tuple(rcv), f0)
+    a, b = np.array(Pold, np.float64), np.array(P0, np.float64)
+    new = np.zeros_like(b)
+    Cf = np.ascontiguousarray(Cf, np.float64)
+    tr = np.zeros(nt)
+    for it in range(nt):
+        tr[it] = lib().orc_ctcs3d_update(C.byref(c), P(a), P(b), P(new), P(Cf), it0 + it)
+        a, b, new = b, new, a
+    return a, b, tr
 
 
 def point_source_run(N, h, c0, dt, nt, src, rcv, f0=100.0, gamma=(1.0, 0.0)):
