@@ -20,7 +20,8 @@ enum { EPI_RHS = 0, EPI_AXPY = 1, EPI_COMB = 2, EPI_F32C = 3 };
 
 // One tile: columns [x0, x0+BX) x [y0, y0+BY), planes [zb, ze) of the grid
 // described by p (shared by the per-grid kernel and the ensemble kernel).
-template <typename T, int SYS, int BX, int BY, int EPI, int PF = 2>
+// WALLS = false: every axis periodic (no ghost signs, wrap-only index maps).
+template <typename T, int SYS, int BX, int BY, int EPI, int PF = 2, bool WALLS = true>
 __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, int zb,
                                             T (&lds)[2][LinTile<T, SYS, BX, BY>::NS][BY + 6][BX + 6])
 {
@@ -40,11 +41,12 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
     const int ze = min(zb + p.zc, nz);
     const bool active = (x0 + tx < nx) && (y0 + ty < ny);
     const size_t plane = (size_t)nx * ny;
+    const int bcx = WALLS ? p.bc[0] : 0, bcy = WALLS ? p.bc[1] : 0, bcz = WALLS ? p.bc[2] : 0;
 
     // own column
     float sgx = 1.f, sgy = 1.f;
-    const int mx = map_axis(x0 + tx, nx, p.bc[0], sgx);
-    const int my = map_axis(y0 + ty, ny, p.bc[1], sgy);
+    const int mx = map_axis(x0 + tx, nx, bcx, sgx);
+    const int my = map_axis(y0 + ty, ny, bcy, sgy);
     const size_t col = (size_t)my * nx + mx;
 
     // halo cell owned by this thread (x halo for rows 0..BY-1, y halo for cols 0..BX-1)
@@ -63,14 +65,14 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
             hx = colx + 3;
             hy = k < 3 ? k : BY + k;
         }
-        int gx = map_axis(x0 + hx - 3, nx, p.bc[0], hsx);
-        int gy = map_axis(y0 + hy - 3, ny, p.bc[1], hsy);
+        int gx = map_axis(x0 + hx - 3, nx, bcx, hsx);
+        int gy = map_axis(y0 + hy - 3, ny, bcy, hsy);
         hcol = (size_t)gy * nx + gx;
     }
 
     auto zmap = [&](int z, float &s) -> size_t {
         s = 1.f;
-        return (size_t)map_axis(z, nz, p.bc[2], s) * plane;
+        return (size_t)map_axis(z, nz, bcz, s) * plane;
     };
 
     // Lead-plane march.  At iteration j the plane j is staged in LDS (own
@@ -151,7 +153,7 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
 #pragma unroll
                 for (int v = 0; v < NV; ++v) cur.v[v] += S.d.v[v];
             }
-            apply_bc_sign<T, NV>(cur, sgx, sgy, S.zs);
+            if constexpr (WALLS) apply_bc_sign<T, NV>(cur, sgx, sgy, S.zs);
 
             T nxy[NP];
 #pragma unroll
@@ -165,7 +167,7 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
 #pragma unroll
                         for (int v = 0; v < NV; ++v) hcur.v[v] += S.hd.v[v];
                     }
-                    apply_bc_sign<T, NV>(hcur, hsx, hsy, S.zs);
+                    if constexpr (WALLS) apply_bc_sign<T, NV>(hcur, hsx, hsy, S.zs);
 #pragma unroll
                     for (int v = 0; v < NS; ++v) lds[buf][v][hy][hx] = hcur.v[v];
                 }
@@ -283,7 +285,7 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
 // halo loads one plane ahead of the neighbours' interior loads.  The column
 // (4 planes ahead) and halo (1 plane ahead) loads keep their raw words until
 // the end of the iteration, so their latency overlaps the iteration's work.
-template <typename T, int SYS, int BX, int BY, int EPI, bool LATE_EPI = false>
+template <typename T, int SYS, int BX, int BY, int EPI, bool LATE_EPI = false, bool WALLS = true>
 __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int y0, int zb,
                                                    T (&lds)[2][LinTile<T, SYS, BX, BY>::NS][BY + 6][BX + 6])
 {
@@ -303,11 +305,12 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
     const int ze = min(zb + p.zc, nz);
     const bool active = (x0 + tx < nx) && (y0 + ty < ny);
     const size_t plane = (size_t)nx * ny;
+    const int bcx = WALLS ? p.bc[0] : 0, bcy = WALLS ? p.bc[1] : 0, bcz = WALLS ? p.bc[2] : 0;
     const bool has_d = (EPI == EPI_F32C) && D != nullptr;
 
     float sgx = 1.f, sgy = 1.f;
-    const int mx = map_axis(x0 + tx, nx, p.bc[0], sgx);
-    const int my = map_axis(y0 + ty, ny, p.bc[1], sgy);
+    const int mx = map_axis(x0 + tx, nx, bcx, sgx);
+    const int my = map_axis(y0 + ty, ny, bcy, sgy);
     const size_t col = (size_t)my * nx + mx;
 
     int hx = -1, hy = -1;
@@ -325,13 +328,13 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
             hx = colx + 3;
             hy = k < 3 ? k : BY + k;
         }
-        int gx = map_axis(x0 + hx - 3, nx, p.bc[0], hsx);
-        int gy = map_axis(y0 + hy - 3, ny, p.bc[1], hsy);
+        int gx = map_axis(x0 + hx - 3, nx, bcx, hsx);
+        int gy = map_axis(y0 + hy - 3, ny, bcy, hsy);
         hcol = (size_t)gy * nx + gx;
     }
     auto zmap = [&](int z, float &s) -> size_t {
         s = 1.f;
-        return (size_t)map_axis(z, nz, p.bc[2], s) * plane;
+        return (size_t)map_axis(z, nz, bcz, s) * plane;
     };
     struct Raw {
         Cell<T, NV> q, d;
@@ -348,7 +351,7 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
 #pragma unroll
             for (int v = 0; v < NV; ++v) c.v[v] += r.d.v[v];
         }
-        apply_bc_sign<T, NV>(c, sx, sy, r.s);
+        if constexpr (WALLS) apply_bc_sign<T, NV>(c, sx, sy, r.s);
         return c;
     };
 
@@ -365,6 +368,11 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
         issue(r, hcol, zb);
         hv = apply(r, hsx, hsy);
     }
+    // Drain the prologue loads before the march.  Otherwise the queue and
+    // halo registers enter the loop as pending load targets, and the
+    // compiler's conservative merge at the loop header makes every iteration
+    // wait for its own 4-planes-ahead prefetch before the barrier.
+    __builtin_amdgcn_s_waitcnt(0);
 
     const T inv48h = T(p.inv48h), Kc = T(p.K), irho = T(p.invrho), sLLF = T(p.s);
     const T lap = T(p.lapc);
