@@ -113,6 +113,43 @@ __device__ __forceinline__ void store_cell(T *__restrict__ base, size_t idx, con
     }
 }
 
+// Streaming forms of load_cell / store_cell (nontemporal hint: the line is
+// not kept in L2 for reuse), for data that no other tile of the launch reads.
+typedef double acx_dbl2 __attribute__((ext_vector_type(2)));
+typedef float acx_flt4 __attribute__((ext_vector_type(4)));
+
+template <typename T, int NV>
+__device__ __forceinline__ Cell<T, NV> load_cell_nt(const T *__restrict__ base, size_t idx)
+{
+    Cell<T, NV> c;
+    if constexpr (sizeof(T) == 8 && NV == 4) {
+        const acx_dbl2 *p = reinterpret_cast<const acx_dbl2 *>(base + idx * 4);
+        const acx_dbl2 a = __builtin_nontemporal_load(p), b = __builtin_nontemporal_load(p + 1);
+        c.v[0] = a.x; c.v[1] = a.y; c.v[2] = b.x; c.v[3] = b.y;
+    } else if constexpr (sizeof(T) == 4 && NV == 4) {
+        const acx_flt4 a = __builtin_nontemporal_load(reinterpret_cast<const acx_flt4 *>(base + idx * 4));
+        c.v[0] = a.x; c.v[1] = a.y; c.v[2] = a.z; c.v[3] = a.w;
+    } else {
+        c = load_cell<T, NV>(base, idx);
+    }
+    return c;
+}
+
+template <typename T, int NV>
+__device__ __forceinline__ void store_cell_nt(T *__restrict__ base, size_t idx, const Cell<T, NV> &c)
+{
+    if constexpr (sizeof(T) == 8 && NV == 4) {
+        acx_dbl2 *p = reinterpret_cast<acx_dbl2 *>(base + idx * 4);
+        __builtin_nontemporal_store(acx_dbl2{c.v[0], c.v[1]}, p);
+        __builtin_nontemporal_store(acx_dbl2{c.v[2], c.v[3]}, p + 1);
+    } else if constexpr (sizeof(T) == 4 && NV == 4) {
+        __builtin_nontemporal_store(acx_flt4{c.v[0], c.v[1], c.v[2], c.v[3]}, reinterpret_cast<acx_flt4 *>(base + idx * 4));
+    } else {
+        store_cell<T, NV>(base, idx, c);
+    }
+}
+
+
 // Linear-scheme face stencils folded into 7-point cell stencils.
 // With the shipped weights (gamma_hi = 1, gamma_lo = 0) the CWENO4 face
 // averages are q_L(i+1/2) = (q_{i-2} - 6q_{i-1} + 20q_i + 10q_{i+1} - q_{i+2})/24

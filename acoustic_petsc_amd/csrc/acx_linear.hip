@@ -20,26 +20,27 @@
 
 namespace acx {
 
-// PF = 0: centre-plane march (PF = -1: with late epilogue loads); PF >= 1:
-// lead-plane march with loads PF planes ahead.  WALLS = false is the
+// PF = 0: centre-plane march (PF = -1: with late epilogue loads; PF = -2:
+// with the 5-plane halo ring); PF >= 1: lead-plane march with loads PF
+// planes ahead.  WALLS = false is the
 // all-periodic instantiation (no ghost signs).
-template <typename T, int SYS, int BX, int BY, int EPI, int PF, int WPE, bool WALLS>
+template <typename T, int SYS, int BX, int BY, int EPI, int PF, int WPE, bool WALLS, bool NT = false>
 __global__ void __launch_bounds__(BX *BY) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_linear(LinArgs p)
 {
     using Tile = LinTile<T, SYS, BX, BY>;
-    __shared__ T lds[2][Tile::NS][BY + 6][BX + 6];
+    __shared__ T lds[PF == -2 ? 5 : 2][Tile::NS][BY + 6][BX + 6];
     const int tile = xcd_logical_tile(blockIdx.x, gridDim.x);
     const int x0 = (tile % p.ntx) * BX, y0 = ((tile / p.ntx) % p.nty) * BY, zb = (tile / (p.ntx * p.nty)) * p.zc;
     if constexpr (PF <= 0)
-        linear_tile_centre<T, SYS, BX, BY, EPI, (PF < 0), WALLS>(p, x0, y0, zb, lds);
+        linear_tile_centre<T, SYS, BX, BY, EPI, (PF == -1), WALLS, NT>(p, x0, y0, zb, lds);
     else
         linear_tile<T, SYS, BX, BY, EPI, PF, WALLS>(p, x0, y0, zb, lds);
 }
 
 // ---------------------------------------------------------------------------
 
-template <typename T, int SYS, int EPI, int PF, int WPE, int BX, int BY, bool WALLS = false>
+template <typename T, int SYS, int EPI, int PF, int WPE, int BX, int BY, bool WALLS = false, bool NT = false>
 static int launch_linear_v(const LinArgs &a, hipStream_t s)
 {
     LinArgs b = a;
@@ -48,7 +49,7 @@ static int launch_linear_v(const LinArgs &a, hipStream_t s)
     b.ntz = (a.nz + a.zc - 1) / a.zc;
     dim3 blk(BX, BY, 1);
     dim3 grd(b.ntx * b.nty * b.ntz, 1, 1);
-    hipLaunchKernelGGL((k_linear<T, SYS, BX, BY, EPI, PF, WPE, WALLS>), grd, blk, 0, s, b);
+    hipLaunchKernelGGL((k_linear<T, SYS, BX, BY, EPI, PF, WPE, WALLS, NT>), grd, blk, 0, s, b);
     return hip_check(hipGetLastError(), "k_linear launch");
 }
 
@@ -57,17 +58,22 @@ static int launch_linear_v(const LinArgs &a, hipStream_t s)
 // SIMD), 2 planes ahead (2) or 3 planes ahead (3); tile 32 x 16 with loads 2
 // (4), 1 (5) or 3 (6) planes ahead; centre-plane march, tile 64 x 8 (7) or
 // 32 x 16 (8), centre-plane 64 x 8 capped for 4 waves per SIMD (9); centre-plane
-// 64 x 8 with late epilogue loads (10), and capped for 4 waves per SIMD (11).
+// 64 x 8 with late epilogue loads (10), and capped for 4 waves per SIMD (11);
+// centre-plane 16 x 32 (12); centre-plane 32 x 16 with streaming epilogue
+// loads and stores (13); centre-plane 32 x 16 with the halo ring (14), and
+// with streaming epilogue (15); halo ring with loads 2 (16) or 3 (17)
+// planes ahead.
+// Defaults (measured at 256^3 and 384^3): FP64 variant 14, FP32 variant 0.
 // Grids with a wall on any axis take the default variant of their precision
-// (7 for FP64, 0 for FP32) instantiated with ghost signs.
+// instantiated with ghost signs.
 template <typename T, int SYS, int EPI>
 static int launch_linear_t(const LinArgs &a, hipStream_t s)
 {
     if (a.bc[0] != ACX_PERIODIC || a.bc[1] != ACX_PERIODIC || a.bc[2] != ACX_PERIODIC) {
-        if constexpr (sizeof(T) == 8) return launch_linear_v<T, SYS, EPI, 0, 1, 64, 8, true>(a, s);
+        if constexpr (sizeof(T) == 8) return launch_linear_v<T, SYS, EPI, -2, 1, 32, 16, true>(a, s);
         else return launch_linear_v<T, SYS, EPI, 1, 1, 64, 8, true>(a, s);
     }
-    static const int var = env_int("ACX_LIN_VARIANT", sizeof(T) == 8 ? 7 : 0);
+    static const int var = env_int("ACX_LIN_VARIANT", sizeof(T) == 8 ? 14 : 0);
     switch (var) {
     case 0: return launch_linear_v<T, SYS, EPI, 1, 1, 64, 8>(a, s);
     case 1: return launch_linear_v<T, SYS, EPI, 1, 4, 64, 8>(a, s);
@@ -80,6 +86,10 @@ static int launch_linear_t(const LinArgs &a, hipStream_t s)
     case 9: return launch_linear_v<T, SYS, EPI, 0, 4, 64, 8>(a, s);
     case 10: return launch_linear_v<T, SYS, EPI, -1, 1, 64, 8>(a, s);
     case 11: return launch_linear_v<T, SYS, EPI, -1, 4, 64, 8>(a, s);
+    case 12: return launch_linear_v<T, SYS, EPI, 0, 1, 16, 32>(a, s);
+    case 13: return launch_linear_v<T, SYS, EPI, 0, 1, 32, 16, false, true>(a, s);
+    case 14: return launch_linear_v<T, SYS, EPI, -2, 1, 32, 16>(a, s);
+    case 15: return launch_linear_v<T, SYS, EPI, -2, 1, 32, 16, false, true>(a, s);
     default: return launch_linear_v<T, SYS, EPI, 2, 1, 64, 8>(a, s);
     }
 }

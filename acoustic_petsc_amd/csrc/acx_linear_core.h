@@ -285,10 +285,21 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
 // halo loads one plane ahead of the neighbours' interior loads.  The column
 // (4 planes ahead) and halo (1 plane ahead) loads keep their raw words until
 // the end of the iteration, so their latency overlaps the iteration's work.
-template <typename T, int SYS, int BX, int BY, int EPI, bool LATE_EPI = false, bool WALLS = true>
+// NT: epilogue operands and results stream past L2 (nontemporal), leaving L2
+// to the stage input that neighbouring tiles re-read as halo.
+//
+// NB = 2: the plane is staged in a double buffer and its halo is loaded one
+// plane ahead.  NB = 5 (halo ring): the halo of plane z+4 is loaded together
+// with the column's plane z+4, i.e. at the iteration in which the
+// neighbouring tiles load those cells as their interior, so the shared rows
+// are still in L2; it waits in its slot of a 5-plane LDS ring until plane z+4
+// is computed.
+template <typename T, int SYS, int BX, int BY, int EPI, bool LATE_EPI, bool WALLS, bool NT, int NB>
 __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int y0, int zb,
-                                                   T (&lds)[2][LinTile<T, SYS, BX, BY>::NS][BY + 6][BX + 6])
+                                                   T (&lds)[NB][LinTile<T, SYS, BX, BY>::NS][BY + 6][BX + 6])
 {
+    static_assert(NB == 2 || NB == 5, "double buffer or 5-plane halo ring");
+    constexpr bool HRING = NB == 5;
     using Tile = LinTile<T, SYS, BX, BY>;
     constexpr int NV = Tile::NV, NS = Tile::NS;
     static_assert(BX * BY >= Tile::NHALO, "every halo cell of a plane needs a thread of its own");
@@ -363,7 +374,22 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
         qz[d + 3] = apply(r, sgx, sgy);
     }
     Cell<T, NV> hv;
-    if (hx >= 0) {
+    if constexpr (HRING) {
+        // halos of planes zb .. zb+3 into ring slots 0..3 (visible after the
+        // first barrier of the march)
+        if (hx >= 0) {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                if (zb + d < ze) {
+                    Raw r;
+                    issue(r, hcol, zb + d);
+                    const Cell<T, NV> c = apply(r, hsx, hsy);
+#pragma unroll
+                    for (int v = 0; v < NS; ++v) lds[d][v][hy][hx] = c.v[v];
+                }
+            }
+        }
+    } else if (hx >= 0) {
         Raw r;
         issue(r, hcol, zb);
         hv = apply(r, hsx, hsy);
@@ -377,15 +403,17 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
     const T inv48h = T(p.inv48h), Kc = T(p.K), irho = T(p.invrho), sLLF = T(p.s);
     const T lap = T(p.lapc);
 
-    int buf = 0;
+    int buf = 0;                      // slot of plane z
+    int hslot = HRING ? 4 : 0;        // ring slot of plane z + 4
     for (int z = zb; z < ze; ++z) {
-        Raw qn, hn;
-        issue(qn, col, z + 4);
-        if (hx >= 0) issue(hn, hcol, z + 1);
+        // column plane z+4; halo plane z+4 (ring) or z+1
+        Raw cq, ch;
+        issue(cq, col, z + 4);
+        if (HRING ? (hx >= 0 && z + 4 < ze) : (hx >= 0)) issue(ch, hcol, HRING ? z + 4 : z + 1);
         const size_t idx = (size_t)z * plane + col;
         Cell<T, NV> e0, e1, e2;
         auto epi_loads = [&]() {
-            if constexpr (EPI == EPI_COMB) e0 = load_cell<T, NV>(U0, idx);
+            if constexpr (EPI == EPI_COMB) e0 = NT ? load_cell_nt<T, NV>(U0, idx) : load_cell<T, NV>(U0, idx);
             if constexpr (EPI == EPI_F32C) {
                 if (has_d) e0 = load_cell<T, NV>(D, idx);
                 if (p.final_stage) {
@@ -399,12 +427,11 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
         if (!LATE_EPI && active) epi_loads();
 #pragma unroll
         for (int v = 0; v < NS; ++v) lds[buf][v][ty + 3][tx + 3] = qz[3].v[v];
-        if (hx >= 0) {
+        if (!HRING && hx >= 0) {
 #pragma unroll
             for (int v = 0; v < NS; ++v) lds[buf][v][hy][hx] = hv.v[v];
         }
         __syncthreads();
-
         Cell<T, NV> r;
         if constexpr (SYS == ACX_ACOUSTIC4) {
             T accB[4], dAp[3], dAvel = T(0);
@@ -477,14 +504,27 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
                     o = nh;
                 }
             }
-            store_cell<T, NV>(OUT, idx, o);
+            if constexpr (NT) store_cell_nt<T, NV>(OUT, idx, o);
+            else store_cell<T, NV>(OUT, idx, o);
         }
 
 #pragma unroll
         for (int k = 0; k < 6; ++k) qz[k] = qz[k + 1];
-        qz[6] = apply(qn, sgx, sgy);
-        if (hx >= 0) hv = apply(hn, hsx, hsy);
-        buf ^= 1;
+        qz[6] = apply(cq, sgx, sgy);
+        if constexpr (HRING) {
+            // slot of plane z+4 = slot of plane z-1, whose readers all passed
+            // this iteration's barrier
+            if (hx >= 0 && z + 4 < ze) {
+                const Cell<T, NV> c = apply(ch, hsx, hsy);
+#pragma unroll
+                for (int v = 0; v < NS; ++v) lds[hslot][v][hy][hx] = c.v[v];
+            }
+            buf = buf == NB - 1 ? 0 : buf + 1;
+            hslot = hslot == NB - 1 ? 0 : hslot + 1;
+        } else {
+            if (hx >= 0) hv = apply(ch, hsx, hsy);
+            buf ^= 1;
+        }
     }
 }
 

@@ -104,7 +104,7 @@ def _host_cpu_model():
     return "unknown"
 
 
-def cpu_baseline(acx, n_sample=96, max_steps=8, budget_s=10.0):
+def cpu_baseline(acx, n_sample=96, max_steps=24, budget_s=10.0):
     """The FP64 oracle (reference loop structure: ghosted copy, per-Gauss-point
     scratch, per-Gauss-point LLF) timed on this host on a 96^3 sample of the
     same workload, on all available cores (up to 16) and on one core."""
