@@ -223,10 +223,10 @@ class SelectiveStepper:
     full CWENO4 where set.
 
     Defaults: at CFL 0.4 a front moves 0.4/3 cell per step, so flags taken
-    every 4 steps and dilated by 2 cells still cover it (0.53 cell) with a
-    margin; the indicator then runs once per 4 steps."""
+    every 8 steps and dilated by 2 cells still cover it (1.07 cells) with a
+    margin; the indicator then runs once per 8 steps."""
 
-    def __init__(self, grid, packed, U64, thr=0.3, refresh=4, dilate=2):
+    def __init__(self, grid, packed, U64, thr=0.3, refresh=8, dilate=2):
         import torch
         self.grid, self.packed, self.thr, self.refresh, self.dilate = grid, packed, thr, refresh, dilate
         self.state = CompensatedState(grid, U64)
