@@ -550,6 +550,11 @@ struct _p_PetscViewer {
     PetscFileMode mode;
     PetscViewerFormat fmt;
     std::string name;
+    // VTK XML structured-grid viewer (PetscViewerVTKOpen): the grid and the
+    // viewed vectors are collected and written when the viewer is destroyed
+    bool vts = false;
+    DM vts_dm = nullptr;
+    std::vector<std::pair<std::string, std::vector<double>>> vts_arrays;
 };
 
 static void put_be32(FILE *fp, int32_t v)
@@ -620,18 +625,85 @@ PetscErrorCode PetscViewerPopFormat(PetscViewer viewer)
     return 0;
 }
 
+PetscErrorCode PetscViewerVTKOpen(MPI_Comm comm, const char *name, PetscFileMode mode, PetscViewer *viewer)
+{
+    (void)comm;
+    if (mode != FILE_MODE_WRITE) return PetscLiteError(56, "PetscViewerVTKOpen: FILE_MODE_WRITE only");
+    PetscErrorCode e = viewer_open(name, true, mode, viewer);
+    if (e) return e;
+    (*viewer)->vts = true;
+    return 0;
+}
+
+// VTK XML StructuredGrid (.vts), appended raw little-endian data with UInt64
+// block headers: Position (3 x Float64 per point; DMDA coordinates when set,
+// else the index), then one Float64 point array per viewed vector component.
+static PetscErrorCode write_vts(PetscViewer v)
+{
+    DM da = v->vts_dm;
+    if (!da && v->vts_arrays.empty()) return 0;
+    if (!da) return PetscLiteError(73, "%s: VecView on a VTK viewer needs a DMView first", v->name.c_str());
+    const int nx = da->M[0], ny = da->dim > 1 ? da->M[1] : 1, nz = da->dim > 2 ? da->M[2] : 1;
+    const size_t np = (size_t)nx * ny * nz;
+    std::vector<double> pos(3 * np, 0.0);
+    const double *c = nullptr;
+    if (da->coords) {
+        PetscErrorCode e = to_host(da->coords);
+        if (e) return e;
+        c = da->coords->host;
+    }
+    for (size_t q = 0; q < np; ++q) {
+        pos[3 * q] = (double)(q % nx);
+        pos[3 * q + 1] = (double)((q / nx) % ny);
+        pos[3 * q + 2] = (double)(q / ((size_t)nx * ny));
+        if (c)
+            for (int d = 0; d < da->dim; ++d) pos[3 * q + d] = c[q * da->dim + d];
+    }
+    FILE *fp = v->fp;
+    const uint16_t probe = 1;
+    if (*(const unsigned char *)&probe != 1) return PetscLiteError(56, "VTK writer: little-endian host required");
+    fprintf(fp, "<?xml version=\"1.0\"?>\n");
+    fprintf(fp, "<VTKFile type=\"StructuredGrid\" version=\"1.0\" byte_order=\"LittleEndian\" header_type=\"UInt64\">\n");
+    fprintf(fp, "  <StructuredGrid WholeExtent=\"0 %d 0 %d 0 %d\">\n", nx - 1, ny - 1, nz - 1);
+    fprintf(fp, "    <Piece Extent=\"0 %d 0 %d 0 %d\">\n", nx - 1, ny - 1, nz - 1);
+    uint64_t off = 0;
+    fprintf(fp, "      <Points>\n        <DataArray type=\"Float64\" Name=\"Position\" NumberOfComponents=\"3\" "
+                "format=\"appended\" offset=\"%llu\"/>\n      </Points>\n", (unsigned long long)off);
+    off += 8 + 8 * pos.size();
+    fprintf(fp, "      <PointData>\n");
+    for (const auto &a : v->vts_arrays) {
+        fprintf(fp, "        <DataArray type=\"Float64\" Name=\"%s\" NumberOfComponents=\"1\" format=\"appended\" "
+                    "offset=\"%llu\"/>\n", a.first.c_str(), (unsigned long long)off);
+        off += 8 + 8 * a.second.size();
+    }
+    fprintf(fp, "      </PointData>\n    </Piece>\n  </StructuredGrid>\n  <AppendedData encoding=\"raw\">\n_");
+    auto block = [&](const std::vector<double> &d) {
+        const uint64_t nb = 8 * d.size();
+        return fwrite(&nb, 8, 1, fp) == 1 && fwrite(d.data(), 8, d.size(), fp) == d.size();
+    };
+    bool ok = block(pos);
+    for (const auto &a : v->vts_arrays) ok = ok && block(a.second);
+    fprintf(fp, "\n  </AppendedData>\n</VTKFile>\n");
+    return ok ? 0 : PetscLiteError(66, "%s: write failed", v->name.c_str());
+}
+
 PetscErrorCode PetscViewerDestroy(PetscViewer *viewer)
 {
     if (!viewer || !*viewer) return 0;
+    PetscErrorCode e = (*viewer)->vts ? write_vts(*viewer) : 0;
     if ((*viewer)->fp) fclose((*viewer)->fp);
     delete *viewer;
     *viewer = nullptr;
-    return 0;
+    return e;
 }
 
 PetscErrorCode DMView(DM da, PetscViewer viewer)
 {
     if (!da || !viewer) return PetscLiteError(85, "DMView: null argument");
+    if (viewer->vts) {
+        viewer->vts_dm = da;
+        return 0;
+    }
     if (viewer->binary || viewer->fmt != PETSC_VIEWER_ASCII_VTK) {
         if (!viewer->binary)
             fprintf(viewer->fp, "DM Object: 1 MPI process\n  type: da\nDMDA %d x %d x %d, dof %d, stencil width %d\n",
@@ -667,6 +739,21 @@ PetscErrorCode VecView(Vec v, PetscViewer viewer)
     PetscErrorCode e = to_host(v);
     if (e) return e;
     FILE *fp = viewer->fp;
+    if (viewer->vts) {
+        DM da = viewer->vts_dm ? viewer->vts_dm : v->dm;
+        if (!da) return PetscLiteError(73, "VecView: VTK output needs a vector from a DMDA");
+        const int dof = da->dof;
+        const size_t np = v->n / dof;
+        for (int f = 0; f < dof; ++f) {
+            const bool named = f < (int)da->fields.size() && !da->fields[f].empty();
+            std::vector<double> d(np);
+            for (size_t q = 0; q < np; ++q) d[q] = v->host[q * dof + f];
+            viewer->vts_arrays.emplace_back(named ? da->fields[f] : std::string("comp") + std::to_string(f),
+                                            std::move(d));
+        }
+        if (!viewer->vts_dm) viewer->vts_dm = da;
+        return 0;
+    }
     if (viewer->binary) {
         if (viewer->mode == FILE_MODE_READ) return PetscLiteError(56, "VecView: viewer opened for reading");
         if (v->n > 0x7fffffff) return PetscLiteError(56, "VecView: vector longer than 2^31 - 1");

@@ -156,6 +156,10 @@ typedef enum { PETSC_VIEWER_DEFAULT, PETSC_VIEWER_ASCII_VTK } PetscViewerFormat;
 #define VEC_FILE_CLASSID 1211214
 PetscErrorCode PetscViewerBinaryOpen(MPI_Comm comm, const char *name, PetscFileMode mode, PetscViewer *viewer);
 PetscErrorCode PetscViewerASCIIOpen(MPI_Comm comm, const char *name, PetscViewer *viewer);
+/* VTK XML structured grid (.vts): DMView records the grid, VecView the
+ * vector's components; the file is written by PetscViewerDestroy
+ * (3d/ctcs/point_source/wave.c:219-231). */
+PetscErrorCode PetscViewerVTKOpen(MPI_Comm comm, const char *name, PetscFileMode mode, PetscViewer *viewer);
 PetscErrorCode PetscViewerPushFormat(PetscViewer viewer, PetscViewerFormat format);
 PetscErrorCode PetscViewerPopFormat(PetscViewer viewer);
 PetscErrorCode PetscViewerDestroy(PetscViewer *viewer);

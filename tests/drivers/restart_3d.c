@@ -9,7 +9,8 @@
  *            DIR/restart1.bin every restart_interval steps (DIR/restart2.bin
  *            every restart_interval + 7 steps), as MonitorFunction does; at
  *            the end DIR/final.bin (VecView, binary) and
- *            DIR/plot/sol-%08d.vtk (DMView + VecView, ASCII VTK).
+ *            DIR/plot/sol-%08d.vtk (DMView + VecView, ASCII VTK) and
+ *            DIR/plot/sol-%08d.vts (PetscViewerVTKOpen, XML structured grid).
  *   resume : VecLoad of DIR/restart1.bin, which holds step
  *            k = restart_interval * floor(steps / restart_interval), then
  *            TSSolve from step k to step `steps`; writes DIR/final_resumed.bin.
@@ -158,6 +159,12 @@ static PetscErrorCode run(int N, int steps, int ri, int wi, int resume, const ch
         snprintf(name, sizeof(name), "%s/plot/sol-%08d.vtk", dir, nsteps);
         CHKERRQ(PetscViewerASCIIOpen(PETSC_COMM_WORLD, name, &viewer));
         CHKERRQ(PetscViewerPushFormat(viewer, PETSC_VIEWER_ASCII_VTK));
+        CHKERRQ(DMView(da, viewer));
+        CHKERRQ(VecView(U, viewer));
+        CHKERRQ(PetscViewerDestroy(&viewer));
+        /* XML structured grid, as write_vts of 3d/ctcs/point_source/wave.c:219-231 */
+        snprintf(name, sizeof(name), "%s/plot/sol-%08d.vts", dir, nsteps);
+        CHKERRQ(PetscViewerVTKOpen(PETSC_COMM_WORLD, name, FILE_MODE_WRITE, &viewer));
         CHKERRQ(DMView(da, viewer));
         CHKERRQ(VecView(U, viewer));
         CHKERRQ(PetscViewerDestroy(&viewer));

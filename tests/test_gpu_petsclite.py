@@ -9,6 +9,7 @@ against each other and against the oracle.
 """
 import json
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -121,3 +122,21 @@ def test_restart_files_and_vtk(tmp_path):
     assert lines[k + 2] == "LOOKUP_TABLE default"
     vals = np.array([[float(t) for t in ln.split()] for ln in lines[k + 3:k + 3 + N ** 3]]).ravel()
     assert np.array_equal(vals, final)
+    # XML structured grid (PetscViewerVTKOpen): header, then appended raw blocks
+    # (UInt64 byte count + little-endian Float64 data) at the declared offsets
+    raw = (tmp_path / "plot" / f"sol-{steps:08d}.vts").read_bytes()
+    head, _, rest = raw.partition(b'<AppendedData encoding="raw">')
+    head = head.decode()
+    assert '<VTKFile type="StructuredGrid"' in head and 'header_type="UInt64"' in head
+    assert f'WholeExtent="0 {N - 1} 0 {N - 1} 0 {N - 1}"' in head
+    names = re.findall(r'Name="([^"]+)"', head)
+    offs = [int(o) for o in re.findall(r'offset="(\d+)"', head)]
+    assert names == ["Position", "P", "U", "V", "W"]
+    data = rest[rest.index(b"_") + 1:]
+    blocks = []
+    for o in offs:
+        nb = int(np.frombuffer(data[o:o + 8], "<u8")[0])
+        blocks.append(np.frombuffer(data[o + 8:o + 8 + nb], "<f8"))
+    assert np.max(np.abs(blocks[0].reshape(-1, 3) - want)) < 1e-14
+    for f in range(4):
+        assert np.array_equal(blocks[1 + f], final[f::4])
