@@ -348,3 +348,29 @@ def test_transmissive_walls(orc, gamma):
     gall = orc.grid3d(n, 0.1, orc.ACOUSTIC4, gamma, bc=(orc.TRANSMISSIVE,) * 3)
     C0 = np.full((n[2], n[1], n[0], 4), 2.5e-4)
     assert np.max(np.abs(orc.rhs3d(gall, C0))) < 1e-15
+
+
+def test_point_source_rhs_is_rhs_plus_one_delta(orc):
+    """The point-source RHS (3d/weno4/point_source/rhs_function.c:337-346) is the
+    plain RHS plus Source(t)/h^3 on component 1 of the source cell only, with
+    Source the Ricker derivative -2 (t - t0) f0^2 exp(-f0^2 (t - t0)^2), t0 = 4/f0
+    (source.c:11-17), and each call records (t, u at the receiver cell)."""
+    N, h, f0 = 12, 0.5, 100.0
+    g = orc.grid3d(N, h, orc.WAVE2, c0=250.0)
+    U = np.random.default_rng(5).standard_normal((N, N, N, 2))
+    F0 = orc.rhs3d(g, U)
+    tt, tu = np.zeros(4), np.zeros(4)
+    src, rcv = (3, 4, 5), (7, 8, 9)
+    ps = orc.PS(g, src, rcv, f0, orc.P(tt), orc.P(tu), 0, 4)
+    for q, t in enumerate((0.0373, 0.04, 0.0)):
+        F = np.zeros_like(U)
+        assert orc.lib().orc_rhs3d_cb_point_source(C.cast(C.pointer(ps), C.c_void_p), t, orc.P(U), orc.P(F)) == 0
+        s = -2.0 * (t - 4.0 / f0) * f0 ** 2 * math.exp(-(f0 ** 2) * (t - 4.0 / f0) ** 2)
+        D = F - F0
+        k = (src[2], src[1], src[0], 1)
+        mask = np.ones(U.shape, bool)
+        mask[k] = False
+        assert np.all(D[mask] == 0.0)                    # same RHS everywhere else
+        # one rounding of F0 + s/h^3 at the source cell
+        assert abs(D[k] - s / h ** 3) <= 4.5e-16 * (abs(F0[k]) + abs(s) / h ** 3)
+        assert ps.ntrace == q + 1 and tt[q] == t and tu[q] == U[rcv[2], rcv[1], rcv[0], 0]
