@@ -81,7 +81,8 @@ struct CwLds {
     static constexpr size_t bytes = (ring + 2ull * NS * BY * BX) * sizeof(T);
 };
 
-template <typename T, int SYS, int BX, int BY, int EPI, bool SEL>
+// WALLS = false: every axis periodic (wrap-only index maps, no ghost signs).
+template <typename T, int SYS, int BX, int BY, int EPI, bool SEL, bool WALLS>
 __global__ void __launch_bounds__(BX *BY)
 k_cweno(CwArgs p)
 {
@@ -114,7 +115,7 @@ k_cweno(CwArgs p)
     const int zb = (tile / (p.ntx * p.nty)) * p.zc, ze = min(zb + p.zc, nz);
     const bool interior = tx >= 1 && tx <= BX - 2 && ty >= 1 && ty <= BY - 2 && gx < nx && gy < ny;
     const size_t plane = (size_t)nx * ny;
-    const int kx = p.bc[0], ky = p.bc[1], kz = p.bc[2];   // boundary kinds
+    const int kx = WALLS ? p.bc[0] : 0, ky = WALLS ? p.bc[1] : 0, kz = WALLS ? p.bc[2] : 0;   // boundary kinds
 
     // own column (mapped) and its ghost sign
     float sx0 = 1.f, sy0 = 1.f;
@@ -169,7 +170,7 @@ k_cweno(CwArgs p)
 #pragma unroll
                     for (int v = 0; v < NV; ++v) c.v[v] += buf[r].d.v[v];
                 }
-                apply_bc_sign<T, NV>(c, rsx[r], rsy[r], buf[r].s);
+                if constexpr (WALLS) apply_bc_sign<T, NV>(c, rsx[r], rsy[r], buf[r].s);
 #pragma unroll
                 for (int v = 0; v < NS; ++v) ring[sl][v][idx / LX][idx % LX] = c.v[v];
             }
@@ -430,8 +431,8 @@ int launch_cweno_tilemask(const CwArgs &a, const uint8_t *flags, uint8_t *mask, 
     return hip_check(hipGetLastError(), "tile mask launch");
 }
 
-template <typename T, int SYS, int EPI, bool SEL>
-static int launch_cw_t(const CwArgs &a, hipStream_t s)
+template <typename T, int SYS, int EPI, bool SEL, bool WALLS>
+static int launch_cw_tw(const CwArgs &a, hipStream_t s)
 {
     constexpr int BX = CW_BX, BY = CW_BY;
     CwArgs b = a;
@@ -442,7 +443,7 @@ static int launch_cw_t(const CwArgs &a, hipStream_t s)
     constexpr size_t lds = CwLds<T, NS, BX, BY>::bytes;
     static bool attr_set = false;
     if (!attr_set) {
-        int rc = hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_cweno<T, SYS, BX, BY, EPI, SEL>),
+        int rc = hip_check(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_cweno<T, SYS, BX, BY, EPI, SEL, WALLS>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                            "k_cweno LDS attribute");
         if (rc) return rc;
@@ -450,8 +451,16 @@ static int launch_cw_t(const CwArgs &a, hipStream_t s)
     }
     dim3 blk(BX, BY, 1);
     dim3 grd(b.ntx * b.nty * b.ntz, 1, 1);
-    hipLaunchKernelGGL((k_cweno<T, SYS, BX, BY, EPI, SEL>), grd, blk, lds, s, b);
+    hipLaunchKernelGGL((k_cweno<T, SYS, BX, BY, EPI, SEL, WALLS>), grd, blk, lds, s, b);
     return hip_check(hipGetLastError(), "k_cweno launch");
+}
+
+template <typename T, int SYS, int EPI, bool SEL>
+static int launch_cw_t(const CwArgs &a, hipStream_t s)
+{
+    if (a.bc[0] != ACX_PERIODIC || a.bc[1] != ACX_PERIODIC || a.bc[2] != ACX_PERIODIC)
+        return launch_cw_tw<T, SYS, EPI, SEL, true>(a, s);
+    return launch_cw_tw<T, SYS, EPI, SEL, false>(a, s);
 }
 
 int launch_cweno(const CwArgs &a, int sys, int epi, bool fp32, bool selective, hipStream_t s)
