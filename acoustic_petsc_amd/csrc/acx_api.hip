@@ -39,10 +39,14 @@ static int check_grid(const acx_grid *g)
 {
     if (!g) return set_error(ACX_EINVAL, "grid is NULL");
     for (int d = 0; d < 3; ++d) {
-        if (g->n[d] < 4) return set_error(ACX_EINVAL, "n[%d] = %d: at least 4 cells per axis are required", d, g->n[d]);
         if (g->bc[d] < ACX_PERIODIC || g->bc[d] > ACX_TRANSMISSIVE)
             return set_error(ACX_EINVAL, "bc[%d] = %d is not a boundary kind", d, g->bc[d]);
+        // n[2] = 1 with a periodic z axis is a 2D grid (one plane)
+        if (d == 2 && g->n[2] == 1 && g->bc[2] == ACX_PERIODIC) continue;
+        if (g->n[d] < 4) return set_error(ACX_EINVAL, "n[%d] = %d: at least 4 cells per axis are required", d, g->n[d]);
     }
+    if (g->n[2] == 1 && !(g->gamma_hi == 1.0 && g->gamma_lo == 0.0))
+        return set_error(ACX_EUNSUPPORTED, "2D grid (n[2] = 1): the shipped linear weights only (gamma = 1, 0)");
     if ((size_t)g->n[0] * g->n[1] > 0x7fffffffu) return set_error(ACX_EINVAL, "plane too large");
     if (g->system != ACX_WAVE2 && g->system != ACX_ACOUSTIC4)
         return set_error(ACX_EINVAL, "system %d unknown", g->system);
@@ -166,8 +170,9 @@ int acx_version(void) { return 1; }
 
 double acx_cfl_dt(const acx_grid *g, double cfl)
 {
-    // dt = CFL h / (3 s_max), s_max = max(c0, c0) = c0 (LinearisedEuler/pde.c:66,87)
-    return (cfl * g->h) / (3.0 * g->c0);
+    // dt = CFL h / (D s_max), s_max = max(c0, c0) = c0 (LinearisedEuler/pde.c:66,87);
+    // D = 3, or 2 on a 2D grid (2d/weno4/bubble/rhs_function.c:385)
+    return (cfl * g->h) / ((g->n[2] == 1 ? 2.0 : 3.0) * g->c0);
 }
 
 int acx_rhs_f64(const acx_grid *g, const double *U, double *RHS, void *stream)

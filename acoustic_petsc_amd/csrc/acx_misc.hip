@@ -247,7 +247,7 @@ int acx_energy_f64(const acx_grid *g, const double *U, double *out, void *stream
     if (rc) return rc;
     double r[3];
     if ((rc = finish(part, 2, r, 3, s))) return rc;
-    out[0] = 0.5 * r[0] * g->h * g->h * g->h;
+    out[0] = 0.5 * r[0] * g->h * g->h * (g->n[2] == 1 ? 1.0 : g->h);   // cell area on a 2D grid
     return ACX_OK;
 }
 

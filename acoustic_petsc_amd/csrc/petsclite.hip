@@ -987,11 +987,12 @@ extern "C" PetscErrorCode TSSolve(TS ts, Vec u)
 
 static PetscErrorCode grid_from_ctx(DM da, const AppCtx *c, acx_grid *g)
 {
-    if (da->dim != 3) return PetscLiteError(56, "RHSFunction: 3D DMDA expected");
+    if (da->dim != 3 && da->dim != 2) return PetscLiteError(56, "RHSFunction: 2D or 3D DMDA expected");
     if (da->s < 3) return PetscLiteError(62, "RHSFunction: the DMDA stencil width must be at least 3 (hype.c:104)");
     memset(g, 0, sizeof(*g));
-    for (int d = 0; d < 3; ++d) g->n[d] = da->M[d];
-    if (c->N_x != da->M[0] || c->N_y != da->M[1] || c->N_z != da->M[2])
+    // a 2D DMDA (2d/weno4/standing_wave/hype.c:94-107) is the one-plane grid n[2] = 1, periodic in z
+    for (int d = 0; d < 3; ++d) g->n[d] = d < da->dim ? da->M[d] : 1;
+    if (c->N_x != da->M[0] || c->N_y != da->M[1] || (da->dim == 3 && c->N_z != da->M[2]))
         return PetscLiteError(60, "RHSFunction: AppCtx N_x/N_y/N_z differ from the DMDA");
     g->system = c->system == ACOUSTIC_SYSTEM ? ACX_ACOUSTIC4 : ACX_WAVE2;
     const int want = g->system == ACX_ACOUSTIC4 ? 4 : 2;
@@ -1000,7 +1001,7 @@ static PetscErrorCode grid_from_ctx(DM da, const AppCtx *c, acx_grid *g)
                                         {c->bottom_boundary, c->top_boundary},
                                         {c->back_boundary, c->front_boundary}};
     for (int d = 0; d < 3; ++d) {
-        const bool per = da->b[d] == DM_BOUNDARY_PERIODIC;
+        const bool per = d >= da->dim || da->b[d] == DM_BOUNDARY_PERIODIC;
         if (per) {
             g->bc[d] = ACX_PERIODIC;
             continue;
@@ -1049,7 +1050,8 @@ static PetscErrorCode point_terms_receiver(const AppCtx *c, const acx_grid *g, c
 static PetscErrorCode point_terms_source(const AppCtx *c, const acx_grid *g, double *OUT, double t, double scale)
 {
     if (!c->source) return 0;
-    const double v = scale * c->source(t) / (g->h * g->h * g->h);
+    // Source(t) / h^3, or / h^2 on a 2D grid (2d/weno4/line_source/rhs_function.c:244-246)
+    const double v = scale * c->source(t) / (g->h * g->h * (g->n[2] == 1 ? 1.0 : g->h));
     const int comp = g->system == ACX_WAVE2 ? 1 : 0;   // WAVE2: v_t; ACOUSTIC4: p_t
     int rc = acx_point_add_f64(g, OUT, c->isx, c->isy, c->isz, comp, v, nullptr);
     return rc ? PetscLiteError(76 + rc, "point source: %s", acx_last_error()) : 0;
@@ -1076,7 +1078,7 @@ extern "C" PetscErrorCode RHSFunction(TS ts, PetscReal t, Vec U, Vec RHS, void *
     return 0;
 }
 
-static bool own_operator(TS ts) { return ts->rhs == RHSFunction && ts->dm && ts->dm->dim == 3; }
+static bool own_operator(TS ts) { return ts->rhs == RHSFunction && ts->dm && ts->dm->dim >= 2; }
 
 static PetscErrorCode fused_rks3_step(TS ts, Vec u, double dt, bool *done)
 {

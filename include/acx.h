@@ -40,8 +40,13 @@ extern "C" {
                                  (3d/weno3/LinearisedEuler/rhs_function.c:47-176) */
 #define ACX_TRANSMISSIVE  3   /* ACOUSTIC4: copy of the boundary cell (same file) */
 
+/* 2D grids: n[2] = 1 with bc[2] = ACX_PERIODIC is one plane, the DMDA of the
+ * reference's 2D programs.  With the shipped linear weights the operator on
+ * it equals the 2D WENO4 operator (2d/weno4/standing_wave/rhs_function.c:12-257
+ * for ACX_WAVE2; for ACX_ACOUSTIC4 with w = 0 the 2D linearised acoustics
+ * (p', u, v), w staying 0); other weights are rejected on 2D grids. */
 typedef struct acx_grid {
-    int32_t n[3];        /* cells along x, y, z                                */
+    int32_t n[3];        /* cells along x, y, z (every axis >= 4, or n[2] = 1) */
     int32_t system;      /* ACX_WAVE2 or ACX_ACOUSTIC4                         */
     int32_t bc[3];       /* boundary kind per axis (both faces of the axis)   */
     int32_t pad_;
@@ -92,11 +97,12 @@ int acx_norms_f64(const double *X, size_t ncells, int nvar, int comp, double *ou
 int acx_diff_f32c_f64(const float *HI, const float *LO, const double *B, size_t n,
                       double *out, void *stream);
 
-/* acoustic energy 1/2 sum (p^2/K + rho0 |u|^2) h^3 (ACOUSTIC4) */
+/* acoustic energy 1/2 sum (p^2/K + rho0 |u|^2) h^3 (ACOUSTIC4; h^2 on a 2D grid) */
 int acx_energy_f64(const acx_grid *g, const double *U, double *out, void *stream);
 
 /* CFL time step, 3d/weno3/LinearisedEuler/rhs_function.c:382-384:
- * CFL*h/(3*s_max), s_max = c0 for the acoustic LLF flux. */
+ * CFL*h/(3*s_max), s_max = c0 for the acoustic LLF flux; CFL*h/(2*s_max) on
+ * a 2D grid (2d/weno4/bubble/rhs_function.c:385). */
 double acx_cfl_dt(const acx_grid *g, double cfl);
 
 /* ---- CTCS leapfrog sibling, 3d/ctcs/point_source/wave.c:138-215 ------- */

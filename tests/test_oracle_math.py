@@ -374,3 +374,24 @@ def test_point_source_rhs_is_rhs_plus_one_delta(orc):
         # one rounding of F0 + s/h^3 at the source cell
         assert abs(D[k] - s / h ** 3) <= 4.5e-16 * (abs(F0[k]) + abs(s) / h ** 3)
         assert ps.ntrace == q + 1 and tt[q] == t and tu[q] == U[rcv[2], rcv[1], rcv[0], 0]
+
+
+@pytest.mark.parametrize("system,bc", [(0, (0, 0)), (0, (1, 1)), (1, (0, 0))])
+def test_2d_operators_are_the_one_plane_3d_operator(orc, system, bc):
+    """With the shipped linear weights the 2D operators (WAVE2 of
+    2d/weno4/standing_wave, 3-variable LLF acoustics) equal the 3D operator on
+    one plane (n_z = 1, periodic z): every z-stencil of a z-invariant field
+    vanishes and the face averages depend only on the 1D five-point
+    combinations along the normal.  ACOUSTIC4 carries w = 0, which stays 0."""
+    N, h = 12, 1.0 / 12
+    nv2 = 2 if system == 0 else 3
+    U2 = np.random.default_rng(3).standard_normal((N, N, nv2))
+    g2 = orc.G2((N, N), h, system, 0, 1.0, 0.0, 1.0, 1.0, bc)
+    R2 = np.zeros_like(U2)
+    assert orc.lib().orc_rhs2d_faithful(C.byref(g2), orc.P(U2), orc.P(R2)) == 0
+    U3 = np.zeros((1, N, N, 2 if system == 0 else 4))
+    U3[0, ..., :nv2] = U2
+    R3 = orc.rhs3d(orc.grid3d((N, N, 1), h, system, (1.0, 0.0), bc=(bc[0], bc[1], 0)), U3)
+    assert np.max(np.abs(R3[0, ..., :nv2] - R2)) <= 1e-14 * np.max(np.abs(R2))
+    if system == 1:
+        assert np.all(R3[..., 3] == 0.0)
