@@ -147,7 +147,8 @@ k_ctcs(const double *__restrict__ Pold, const double *__restrict__ P, double *__
     const int nx = c.n[0], ny = c.n[1], nz = c.n[2];
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
     if (i >= nx) return;
-    auto W = [](int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); };
+    // periodic wrap; an axis of 1 cell (2D / 1D programs) maps every offset to 0
+    auto W = [](int a, int n) { a %= n; return a < 0 ? a + n : a; };
     auto at = [&](int a, int b, int e) { return P[((size_t)W(e, nz) * ny + W(b, ny)) * nx + W(a, nx)]; };
     const double invdx2 = 1.0 / (c.dx * c.dx), invdy2 = 1.0 / (c.dy * c.dy), invdz2 = 1.0 / (c.dz * c.dz);
     const double p0 = at(i, j, k);
@@ -256,7 +257,7 @@ int acx_ctcs_update_f64(const acx_ctcs *c, const double *POLD, const double *P, 
 {
     if (!c || !POLD || !P || !PNEW || !C || PNEW == P) return set_error(ACX_EINVAL, "ctcs: bad arguments");
     for (int d = 0; d < 3; ++d)
-        if (c->n[d] < 5) return set_error(ACX_EINVAL, "ctcs: at least 5 cells per axis");
+        if (c->n[d] < 5 && c->n[d] != 1) return set_error(ACX_EINVAL, "ctcs: at least 5 cells per axis (or 1)");
     if (c->stencil != 1 && c->stencil != 2) return set_error(ACX_EINVAL, "stencil width can be either 1 or 2");
     double *part = scratch_for_device();
     if (!part) return set_error(ACX_EHIP, "ctcs: scratch allocation failed");

@@ -116,7 +116,10 @@ typedef struct acx_ctcs {
     double  f0;          /* Ricker-derivative dominant frequency                */
 } acx_ctcs;
 
-/* PNEW = 2P - POLD + c^2 dt^2 Lap(P) + source(it*dt)*dt^2/(dx dy dz) at src;
+/* One axis of 1 cell makes the 2D and 1D programs (2d/ctcs/plane_wave/wave.c:161-202,
+ * 1d/ctcs/travelling_wave/wave.c:170-212): its second difference vanishes;
+ * set its spacing to 1 so that the source divides by dx dy (or dx).
+ * PNEW = 2P - POLD + c^2 dt^2 Lap(P) + source(it*dt)*dt^2/(dx dy dz) at src;
  * C is the per-cell wave speed (device); *rcv_out (host, may be NULL) gets
  * P at the receiver before the update. */
 int acx_ctcs_update_f64(const acx_ctcs *c, const double *POLD, const double *P, double *PNEW,

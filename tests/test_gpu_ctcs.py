@@ -84,3 +84,33 @@ def test_fields_match_oracle(acx, orc, torch_gpu, n, d, stencil):
         assert np.linalg.norm(g - o) / np.linalg.norm(o) < 1e-13
         assert np.max(np.abs(g - o)) < 1e-12 * np.max(np.abs(o))
     assert np.max(np.abs(gtr - otr)) < 1e-12 * np.max(np.abs(otr))
+
+
+@pytest.mark.parametrize("n", [(64, 64, 1), (128, 1, 1)], ids=["2d-plane-wave", "1d-travelling-wave"])
+def test_2d_1d_programs(acx, orc, torch_gpu, n):
+    """The 2D and 1D CTCS programs (2d/ctcs/plane_wave/wave.c:28-117,161-202,
+    1d/ctcs/travelling_wave/wave.c) as grids with axes of one cell:
+    p = sin(2 pi (x - t)) sampled at the nodes x = i h, dt = 0.5 CFL h / c with
+    CFL 0.5, two exact start levels; the fields match the oracle and the
+    error against the exact wave falls at second order."""
+    errs = []
+    for N in (n[0] // 2, n[0]):
+        nn = (N, 1 if n[1] == 1 else N, 1)
+        h = 1.0 / N
+        dt = 0.5 * 0.5 * h
+        x = np.arange(N) * h
+        shape = (1, nn[1], N)
+        P0 = np.broadcast_to(np.sin(2 * np.pi * x), shape).copy()
+        P1 = np.broadcast_to(np.sin(2 * np.pi * (x - dt)), shape).copy()
+        Cf = np.ones(shape)
+        nt = int(round(0.25 / dt))
+        d = (h, h if nn[1] > 1 else 1.0, 1.0)
+        print(f"[case] ctcs {nn} nt={nt}", flush=True)
+        oa, ob, _ = orc.ctcs_steps(nn, d, dt, 1, (-1, -1, -1), (0, 0, 0), P0, P1, Cf, nt)
+        ga, gb, _ = _gpu_steps(acx, torch_gpu, nn, d, dt, 1, (-1, -1, -1), (0, 0, 0), P0, P1, Cf, nt)
+        assert np.max(np.abs(gb - ob)) < 1e-12
+        exact = np.sin(2 * np.pi * (x - (nt + 1) * dt))
+        errs.append(np.max(np.abs(gb - exact)))
+    rate = np.log2(errs[0] / errs[1])
+    print(f"[case] ctcs {n}: errors {errs[0]:.3e} {errs[1]:.3e} rate {rate:.2f}", flush=True)
+    assert 1.8 < rate < 2.2
