@@ -113,7 +113,6 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
             const size_t cidx = (size_t)c * plane + col;
             if constexpr (EPI == EPI_COMB) S.e0 = load_cell<T, NV>(U0, cidx);
             if constexpr (EPI == EPI_F32C) {
-                if (has_d) S.e0 = load_cell<T, NV>(D, cidx);
                 if (fin_stage) {
                     S.e1 = load_cell<T, NV>(LO, cidx);
                     S.e2 = load_cell<T, NV>(Q, cidx);   // HI of plane c, exact
@@ -124,6 +123,13 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
 
     Cell<T, NV> qz[7];   // column values of planes j-6 .. j after the push
     T xyq[4][NP];        // in-plane RHS parts of planes j-3 .. j
+    // F32C: the increment D of planes j-3 .. j, kept from the stencil loads for
+    // the epilogue of plane j-3 (a second load of it three planes later missed L2)
+    Cell<T, NV> dq[EPI == EPI_F32C ? 4 : 1];
+#pragma unroll
+    for (int k = 0; k < (EPI == EPI_F32C ? 4 : 1); ++k)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) dq[k].v[v] = T(0);
 #pragma unroll
     for (int k = 0; k < 7; ++k)
 #pragma unroll
@@ -152,6 +158,11 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
             if (has_d) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) cur.v[v] += S.d.v[v];
+                if constexpr (EPI == EPI_F32C) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) dq[k] = dq[k + 1];
+                    dq[3] = S.d;
+                }
             }
             if constexpr (WALLS) apply_bc_sign<T, NV>(cur, sgx, sgy, S.zs);
 
@@ -255,7 +266,7 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
                     // compensated FP32: increments relative to the step's base (HI + LO)
                     const T bb = T(p.b), cc = T(p.c);
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) o.v[v] = (has_d ? bb * S.e0.v[v] : T(0)) + cc * r.v[v];
+                    for (int v = 0; v < NV; ++v) o.v[v] = (has_d ? bb * dq[0].v[v] : T(0)) + cc * r.v[v];
                     if (fin_stage) {
                         Cell<T, NV> nh, nl;
 #pragma unroll
