@@ -167,3 +167,35 @@ def test_stage_epilogues(acx, orc, torch_gpu):
         err = rel_l2(out.cpu().numpy(), ref)
         print(f"[case] stage a={a} b={b} c={c} rel_l2={err:.3e}", flush=True)
         assert err <= TOL
+
+
+@pytest.mark.parametrize("system,gamma,bc", [(0, (1.0, 0.0), (0, 0, 0)), (0, (1.0, 0.0), (1, 0, 1)),
+                                             (1, (0.85, 0.85), (0, 0, 0)), (1, (1.0, 0.0), (0, 0, 0))],
+                         ids=["w2-lin-per", "w2-lin-adia", "ac4-nl-per", "ac4-lin-per"])
+def test_f32c_steps_track_fp64_oracle(system, gamma, bc, acx, orc, torch_gpu):
+    """The FP32 compensated stepper (both systems, linear and nonlinear weights,
+    walls) against the FP64 oracle after five steps, at FP32 accuracy."""
+    torch = torch_gpu
+    n = (28, 22, 18)
+    grid = acx.Grid(n=n, h=1.0 / 28, system=system, bc=bc, gamma_hi=gamma[0], gamma_lo=gamma[1])
+    U = make_field(acx, grid, 21, "smooth")
+    dt = 0.4 * grid.h / 3.0
+    print(f"[case] f32c system={system} gamma={gamma} bc={bc}", flush=True)
+    ref = U.copy()
+    g = orc_grid(orc, grid)
+    for _ in range(5):
+        orc.ts_solve("orc_rhs3d_cb_faithful", g, ref, dt, dt, 0, 4)
+    st = acx.CompensatedState(grid, torch.from_numpy(U).cuda())
+    for _ in range(5):
+        st.step(dt)
+    err = rel_l2(st.value64().cpu().numpy(), ref)
+    print(f"[case] f32c system={system} gamma={gamma} bc={bc}: rel_l2 {err:.2e}", flush=True)
+    assert err < 1e-6
+
+
+def test_f32c_wave2_nonlinear_is_refused(acx, torch_gpu):
+    torch = torch_gpu
+    grid = acx.Grid(n=(16, 16, 16), h=1.0 / 16, system=acx.WAVE2, gamma_hi=0.85, gamma_lo=0.85)
+    st = acx.CompensatedState(grid, torch.zeros(grid.shape(), dtype=torch.float64, device="cuda"))
+    with pytest.raises(acx.AcxError, match="ACOUSTIC4"):
+        st.step(1e-3)
