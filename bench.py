@@ -108,6 +108,7 @@ def cpu_baseline(acx, n_sample=96, max_steps=24, budget_s=10.0):
     """The FP64 oracle (reference loop structure: ghosted copy, per-Gauss-point
     scratch, per-Gauss-point LLF) timed on this host on a 96^3 sample of the
     same workload, on all available cores (up to 16) and on one core."""
+    import numpy as np
     from oracle import oracle as orc
     avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     grid = acx.Grid(n=(n_sample,) * 3, h=1.0 / n_sample, system=acx.ACOUSTIC4)
@@ -137,6 +138,17 @@ def cpu_baseline(acx, n_sample=96, max_steps=24, budget_s=10.0):
     # the oracle's streaming face-average form (no scratch, one pass per
     # stage, OpenMP over z-slabs): what a tuned CPU code of the same scheme does
     vs, steps_s, el_s = run(threads, 4 * max_steps, budget_s / 2, U0, g, n_sample, "orc_rhs3d_cb_stream")
+    # the 2D case of the reference's CPU configuration (256^2 linearised
+    # acoustics, CFL 0.4): the oracle's 2D restatement, one core, bounded
+    N2 = 256
+    U2 = np.ascontiguousarray(pulse_2d(N2)[0, :, :, :3])
+    g2 = orc.G2((N2, N2), 1.0 / N2, orc.ACOUSTIC4, 0, 1.0, 0.0, 1.0, 1.0, (0, 0))
+    dt2 = 0.4 * (1.0 / N2) / 2.0
+    steps2, t2 = 0, time.perf_counter()
+    while steps2 < 200 and time.perf_counter() - t2 < budget_s / 2:
+        orc.ts_solve("orc_rhs2d_cb", g2, U2, dt2, dt2, 0, 4)
+        steps2 += 1
+    el2 = time.perf_counter() - t2
     return {"value": v, "unit": "cell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"{n_sample}^3 ACOUSTIC4 FP64, same 4-Gaussian workload, {steps} SSP-RK3 steps "
@@ -144,10 +156,50 @@ def cpu_baseline(acx, n_sample=96, max_steps=24, budget_s=10.0):
             "single_core": {"value": v1, "sample": f"{n1}^3, {steps1} steps, {el1:.1f} s"},
             "streaming": {"value": vs, "cores": threads,
                           "sample": f"{n_sample}^3, {steps_s} steps of the streaming face-average RHS, {el_s:.1f} s"},
+            "acoustic2d_256": {"value": N2 * N2 * STAGES * steps2 / el2, "cores": 1,
+                               "sample": f"256^2 2D linearised acoustics, {steps2} of 200 steps, {el2:.1f} s"},
             "host": {"cpu": _host_cpu_model(), "available_cores": avail, "nproc": os.cpu_count()}}
 
 
-VARIANTS = ("fp64_cweno", "fp32c", "learned", "ensemble")
+VARIANTS = ("fp64_cweno", "fp32c", "learned", "ensemble", "2d")
+
+
+def pulse_2d(N, seed=1001, amp=1e-3, sigma=0.05):
+    """The 2D linearised-acoustics initial state on the periodic unit square,
+    N^2 cells: p' = amp exp(-r^2 / (2 sigma^2)) about a seeded centre (minimum
+    image), as exact cell averages (a product of erf differences, nine
+    periodic images), u = v = 0.  Returned as (1, N, N, 4) with w = 0: the
+    one-plane ACOUSTIC4 grid on which the library runs 2D programs."""
+    import numpy as np
+    from scipy.special import erf
+    c = np.random.default_rng(seed).random(2)
+    h = 1.0 / N
+    edges = np.arange(N + 1) * h
+    k = sigma * np.sqrt(np.pi / 2.0) / h
+    prof = []
+    for d in range(2):
+        acc = np.zeros(N)
+        for img in (-1.0, 0.0, 1.0):
+            z = (edges - c[d] + img) / (sigma * np.sqrt(2.0))
+            acc += erf(z[1:]) - erf(z[:-1])
+        prof.append(k * acc)
+    U = np.zeros((1, N, N, 4))
+    U[0, :, :, 0] = amp * np.outer(prof[1], prof[0])
+    return U
+
+
+def variant_2d(torch, acx, steps=200, N=256):
+    """The 2D case of the reference's CPU configuration: linearised acoustics
+    (p', u, v; rho0 = c0 = 1) on a 256^2 periodic grid, FP64, SSP-RK3 at CFL
+    0.4 (dt = CFL h / 2), 200 steps, one Gaussian pulse, run on the GPU as a
+    one-plane grid."""
+    g = acx.Grid(n=(N, N, 1), h=1.0 / N, system=acx.ACOUSTIC4)
+    U = torch.from_numpy(pulse_2d(N)).cuda()
+    work = (torch.empty_like(U), torch.empty_like(U))
+    dt = acx.cfl_dt(g, 0.4)
+    t, _ = time_steps(torch, lambda: acx.ssp_rks3_step(g, U, dt, work), steps, 2)
+    return {"value": N * N * STAGES * steps / t, "ms_per_step": 1e3 * t / steps, "steps": steps,
+            "grid": [N, N, 1], "note": "launch-bound: 65536 cells per stage launch"}
 
 
 def variant_fp64_cweno(torch, acx, N, U0, work, dt, steps):
@@ -391,6 +443,8 @@ def main():
             var.update(variant_learned(torch, acx, args.learned_sizes, args.steps, var, value))
         if "ensemble" in want:
             var["ensemble_48"] = variant_ensemble(torch, acx, args.steps)
+        if "2d" in want:
+            var["acoustic2d_256"] = variant_2d(torch, acx)
         out["variants"] = var
 
     if not args.no_cpu:
