@@ -106,7 +106,11 @@ k_cweno(CwArgs p)
 
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-    const int tile = xcd_logical_tile(blockIdx.x, gridDim.x);
+    // Selective patch: the few marked tiles cluster in space (along a front),
+    // so they are spread over the XCDs round-robin (block b -> tile b, XCD
+    // b mod 8) instead of giving each XCD a contiguous range of tiles, which
+    // piled the marked tiles onto a few XCDs.
+    const int tile = SEL ? (int)blockIdx.x : xcd_logical_tile(blockIdx.x, gridDim.x);
     if constexpr (SEL) {
         if (p.tilemask && !p.tilemask[tile]) return;   // whole block: the linear pass holds this tile
     }
