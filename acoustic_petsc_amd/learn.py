@@ -223,10 +223,10 @@ class SelectiveStepper:
     full CWENO4 where set.
 
     Defaults: at CFL 0.4 a front moves 0.4/3 cell per step, so flags taken
-    every 8 steps and dilated by 2 cells still cover it (1.07 cells) with a
-    margin; the indicator then runs once per 8 steps."""
+    every 12 steps and dilated by 3 cells still cover it (1.6 cells) with a
+    margin of more than a cell; the indicator then runs once per 12 steps."""
 
-    def __init__(self, grid, packed, U64, thr=0.3, refresh=8, dilate=2):
+    def __init__(self, grid, packed, U64, thr=0.3, refresh=12, dilate=3):
         import torch
         self.grid, self.packed, self.thr, self.refresh, self.dilate = grid, packed, thr, refresh, dilate
         self.state = CompensatedState(grid, U64)

@@ -284,7 +284,7 @@ def variant_learned(torch, acx, sizes, steps, var, fp64_value):
         lin64 = U0.cuda()
         tl64, _ = time_steps(torch, lambda: acx.ssp_rks3_step(glin, lin64, dt, w), k, 1)
         del lin64, w
-        sel = learn.SelectiveStepper(g, packed, U0.cuda(), thr=0.3, refresh=8, dilate=2)
+        sel = learn.SelectiveStepper(g, packed, U0.cuda(), thr=0.3, refresh=12, dilate=3)
         sel.step(dt)
         fr = []
         tsel, _ = time_steps(torch, lambda: sel.step(dt), k, 0)
@@ -306,7 +306,7 @@ def variant_learned(torch, acx, sizes, steps, var, fp64_value):
         vl64 = cells * STAGES * k / tl64
         res[f"fp32_learned_{M}"] = {
             "value": v, "ms_per_step": 1e3 * tsel / k, "steps": k + 1, "grid": [M, M, M],
-            "indicator": {"threshold": 0.3, "refresh_steps": 8, "dilation_cells": 2, "label": "sum|w-gamma| > 0.05"},
+            "indicator": {"threshold": 0.3, "refresh_steps": 12, "dilation_cells": 3, "label": "sum|w-gamma| > 0.05"},
             "flagged_frac_last_step": fr[-1], "skipped_frac_last_step": 1.0 - fr[-1],
             "rel_l2_vs_fp64_weno": rel, "max_abs_vs_fp64_weno": mx,
             "linear_only_rel_l2_vs_fp64_weno": rel_lin,
