@@ -117,6 +117,8 @@ def lib():
             "acx_ssp_rks3_step_f64": [gp, vp, vp, vp, dbl, vp],
             "acx_ssp_rks3_step_f32c": [gp, vp, vp, vp, vp, vp, vp, dbl, vp],
             "acx_ssp_rks3_step_f32c_sel": [gp, vp, vp, vp, vp, vp, vp, vp, dbl, vp],
+            "acx_ssp_rks3_step_f32c_selm": [gp, vp, vp, vp, vp, vp, vp, vp, vp, dbl, vp],
+            "acx_sel_mask": [gp, vp, vp, vp],
             "acx_norms_f64": [vp, _C.c_size_t, i32, i32, _C.POINTER(dbl), vp],
             "acx_diff_f32c_f64": [vp, vp, vp, _C.c_size_t, _C.POINTER(dbl), vp],
             "acx_energy_f64": [gp, vp, _C.POINTER(dbl), vp],
@@ -132,6 +134,8 @@ def lib():
         L.acx_cfl_dt.argtypes = [gp]
         L.acx_cfl_dt.argtypes = [gp, dbl]
         L.acx_cfl_dt.restype = dbl
+        L.acx_sel_mask_bytes.argtypes = [gp]
+        L.acx_sel_mask_bytes.restype = _C.c_size_t
         _lib = L
     return _lib
 
@@ -218,11 +222,18 @@ class CompensatedState:
             self.hi.copy_(U64.to(torch.float32))
             self.lo.copy_((U64 - self.hi.to(torch.float64)).to(torch.float32))
 
-    def step(self, dt, flags=None, stream=None):
+    def step(self, dt, flags=None, mask=None, stream=None):
+        """One step; with flags, the selective step (mask: the tile mask of
+        those flags from sel_mask, else made on every call)."""
         torch = _torch()
         g = self.grid.c_struct()
         f32 = torch.float32
-        if flags is None:
+        if mask is not None:
+            rc = lib().acx_ssp_rks3_step_f32c_selm(_C.byref(g), _ptr(self.hi, f32), _ptr(self.lo, f32),
+                                                  _ptr(self._hi2, f32), _ptr(self._lo2, f32), _ptr(self.d0, f32),
+                                                  _ptr(self.d1, f32), _ptr(flags, torch.uint8),
+                                                  _ptr(mask, torch.uint8), dt, _stream(stream))
+        elif flags is None:
             rc = lib().acx_ssp_rks3_step_f32c(_C.byref(g), _ptr(self.hi, f32), _ptr(self.lo, f32),
                                              _ptr(self._hi2, f32), _ptr(self._lo2, f32), _ptr(self.d0, f32),
                                              _ptr(self.d1, f32), dt, _stream(stream))
@@ -237,6 +248,21 @@ class CompensatedState:
     def value64(self):
         torch = _torch()
         return self.hi.to(torch.float64) + self.lo.to(torch.float64)
+
+
+def sel_mask(grid, flags, mask=None, stream=None):
+    """Tile mask of the selective step for these flags (uint8 CUDA tensor,
+    reused when `mask` has the right size)."""
+    torch = _torch()
+    L = lib()
+    g = grid.c_struct()
+    n = L.acx_sel_mask_bytes(_C.byref(g))
+    if n == 0:
+        raise AcxError(1, "selective mask: invalid grid")
+    if mask is None or mask.numel() != n:
+        mask = torch.empty(n, dtype=torch.uint8, device="cuda")
+    _check(L.acx_sel_mask(_C.byref(g), _ptr(flags, torch.uint8), _ptr(mask, torch.uint8), _stream(stream)))
+    return mask
 
 
 def norms(X, comp=-1, nvar=None, stream=None):

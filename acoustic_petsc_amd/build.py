@@ -25,6 +25,14 @@ HIP_FLAGS = [
 ]
 
 
+# Per-source additions.  acx_learn.hip: MFMA results in VGPRs (the
+# activation step reads them directly instead of copying each one out of the
+# accumulation registers first).
+FILE_FLAGS = {
+    "acx_learn.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
+
+
 def _hipcc():
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and os.path.exists(cand):
@@ -45,7 +53,7 @@ def build(verbose=False, jobs=None):
     procs = []
     for src in srcs:
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
-        cmd = [hipcc, "-c", "-x", "hip", src, "-o", obj] + HIP_FLAGS
+        cmd = [hipcc, "-c", "-x", "hip", src, "-o", obj] + HIP_FLAGS + FILE_FLAGS.get(os.path.basename(src), [])
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

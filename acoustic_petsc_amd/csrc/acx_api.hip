@@ -288,10 +288,12 @@ static uint8_t *sel_mask_scratch(size_t n)
     return buf[dev];
 }
 
+// Patch tiling: 30 x 14 x zc cells, zc read once per process, so a tile mask
+// made by acx_sel_mask stays valid for every later selective step.
 static void fill_cw_patch(CwArgs &a, const acx_grid *g)
 {
     fill_cw(a, g);
-    const int zc = env_int("ACX_ZC_PATCH", 16);
+    static const int zc = env_int("ACX_ZC_PATCH", 4);
     a.zc = zc < 1 ? 1 : (zc > g->n[2] ? g->n[2] : zc);
 }
 
@@ -319,9 +321,8 @@ static int stage32c_sel(const acx_grid *g, const float *HI, const float *D, cons
 } // namespace acx
 
 // selective (learned-indicator) step, declared in acx_learn.h
-extern "C" int acx_ssp_rks3_step_f32c_sel(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
-                                          float *LO_OUT, float *D0, float *D1, const uint8_t *flags,
-                                          double dt, void *stream)
+static int check_sel(const acx_grid *g, const float *HI, const float *LO, const float *HI_OUT, const float *LO_OUT,
+                     const float *D0, const float *D1, const uint8_t *flags)
 {
     int rc = check_grid(g);
     if (rc) return rc;
@@ -329,21 +330,69 @@ extern "C" int acx_ssp_rks3_step_f32c_sel(const acx_grid *g, const float *HI, co
     if (!HI || !LO || !HI_OUT || !LO_OUT || !D0 || !D1 || HI == HI_OUT || LO == LO_OUT)
         return set_error(ACX_EINVAL, "f32c step: outputs must not alias the inputs");
     if (g->system != ACX_ACOUSTIC4) return set_error(ACX_EUNSUPPORTED, "selective step: ACOUSTIC4 only");
-    hipStream_t s = (hipStream_t)stream;
-    CwArgs ca;
-    fill_cw_patch(ca, g);
-    if (env_int("ACX_SEL_FULL", 0)) {
-        // check mode: the CWENO kernel over every tile, no linear pass
+    return ACX_OK;
+}
+
+// The four selective stages given the tile mask (null mask: the check mode
+// ACX_SEL_FULL=1, the CWENO kernel over every tile and no linear pass).
+static int sel_stages(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT, float *LO_OUT, float *D0,
+                      float *D1, const uint8_t *flags, const uint8_t *mask, double dt, hipStream_t s)
+{
+    int rc;
+    if (!mask) {
         if ((rc = stage32c(g, HI, nullptr, LO, D0, nullptr, 0.0, dt / 2, 0, flags, s))) return rc;
         if ((rc = stage32c(g, HI, D0, LO, D1, nullptr, 1.0, dt / 2, 0, flags, s))) return rc;
         if ((rc = stage32c(g, HI, D1, LO, D0, nullptr, 1.0 / 3.0, dt / 6, 0, flags, s))) return rc;
         return stage32c(g, HI, D0, LO, HI_OUT, LO_OUT, 1.0, dt / 2, 1, flags, s);
     }
-    uint8_t *mask = sel_mask_scratch(cweno_sel_tiles(ca));
-    if (!mask) return set_error(ACX_EHIP, "selective step: tile mask allocation failed");
-    if ((rc = launch_cweno_tilemask(ca, flags, mask, s))) return rc;
     if ((rc = stage32c_sel(g, HI, nullptr, LO, D0, nullptr, 0.0, dt / 2, 0, flags, mask, s))) return rc;
     if ((rc = stage32c_sel(g, HI, D0, LO, D1, nullptr, 1.0, dt / 2, 0, flags, mask, s))) return rc;
     if ((rc = stage32c_sel(g, HI, D1, LO, D0, nullptr, 1.0 / 3.0, dt / 6, 0, flags, mask, s))) return rc;
     return stage32c_sel(g, HI, D0, LO, HI_OUT, LO_OUT, 1.0, dt / 2, 1, flags, mask, s);
+}
+
+extern "C" int acx_ssp_rks3_step_f32c_sel(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
+                                          float *LO_OUT, float *D0, float *D1, const uint8_t *flags,
+                                          double dt, void *stream)
+{
+    int rc = check_sel(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (env_int("ACX_SEL_FULL", 0)) return sel_stages(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags, nullptr, dt, s);
+    CwArgs ca;
+    fill_cw_patch(ca, g);
+    uint8_t *mask = sel_mask_scratch(cweno_sel_tiles(ca));
+    if (!mask) return set_error(ACX_EHIP, "selective step: tile mask allocation failed");
+    if ((rc = launch_cweno_tilemask(ca, flags, mask, s))) return rc;
+    return sel_stages(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags, mask, dt, s);
+}
+
+extern "C" size_t acx_sel_mask_bytes(const acx_grid *g)
+{
+    if (check_grid(g)) return 0;
+    CwArgs ca;
+    fill_cw_patch(ca, g);
+    return cweno_sel_tiles(ca);
+}
+
+extern "C" int acx_sel_mask(const acx_grid *g, const uint8_t *flags, uint8_t *mask, void *stream)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!flags || !mask) return set_error(ACX_EINVAL, "selective mask: null pointer");
+    CwArgs ca;
+    fill_cw_patch(ca, g);
+    return launch_cweno_tilemask(ca, flags, mask, (hipStream_t)stream);
+}
+
+extern "C" int acx_ssp_rks3_step_f32c_selm(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
+                                           float *LO_OUT, float *D0, float *D1, const uint8_t *flags,
+                                           const uint8_t *mask, double dt, void *stream)
+{
+    int rc = check_sel(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags);
+    if (rc) return rc;
+    if (!mask) return set_error(ACX_EINVAL, "selective step: mask is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    if (env_int("ACX_SEL_FULL", 0)) return sel_stages(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags, nullptr, dt, s);
+    return sel_stages(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags, mask, dt, s);
 }

@@ -81,9 +81,18 @@ struct CwLds {
     static constexpr size_t bytes = (ring + 2ull * NS * BY * BX) * sizeof(T);
 };
 
+// Waves per SIMD the register allocation aims for: the selective patch runs a
+// few hundred marked tiles, so two blocks per CU (4 waves per SIMD, at most
+// 128 VGPRs) let one launch hold them all at once; the full-grid kernel keeps
+// the registers (one block per CU).
+template <bool SEL>
+struct CwWaves {
+    static constexpr int min_per_simd = SEL ? 4 : 1;
+};
+
 // WALLS = false: every axis periodic (wrap-only index maps, no ghost signs).
 template <typename T, int SYS, int BX, int BY, int EPI, bool SEL, bool WALLS>
-__global__ void __launch_bounds__(BX *BY)
+__global__ void __launch_bounds__(BX *BY) __attribute__((amdgpu_waves_per_eu(CwWaves<SEL>::min_per_simd)))
 k_cweno(CwArgs p)
 {
     constexpr int NV = SYS == ACX_ACOUSTIC4 ? 4 : 2;
@@ -391,32 +400,44 @@ k_cweno(CwArgs p)
 
 constexpr int CW_BX = 32, CW_BY = 16;   // reconstruction columns per block (output tile 30 x 14)
 
-__global__ void k_fill_u8(uint8_t *m, size_t n, uint8_t v)
-{
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) m[i] = v;
-}
+// One block per column of up to TM_ZT CWENO tiles along z: a tile is marked
+// when a cell of it, or a face neighbour of one (a cell of the one-cell ring
+// outside the tile in exactly one coordinate, periodic wrap), is flagged;
+// their outputs use the flagged reconstruction.  Every entry is written (0 or
+// 1), one store per tile.
+constexpr int TM_ZT = 8;
 
-// Marks the tile of every flagged cell and of its six face neighbours
-// (periodic wrap): their outputs use the flagged reconstruction.
-__global__ void k_tile_mark(const uint8_t *__restrict__ flags, int nx, int ny, int nz, int zc, int ntx, int nty,
-                            uint8_t *__restrict__ mask)
+__global__ void __launch_bounds__(256) k_tile_mark(const uint8_t *__restrict__ flags, int nx, int ny, int nz, int zc,
+                                                   int ntx, int nty, int ntz, uint8_t *__restrict__ mask)
 {
-    const size_t n = (size_t)nx * ny * nz;
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n || flags[c] == 0) return;
-    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
-    auto mark = [&](int x, int y, int z) {
-        x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
-        y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
-        z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
-        const size_t t = ((size_t)(z / zc) * nty + y / (CW_BY - 2)) * ntx + x / (CW_BX - 2);
-        mask[t] = 1;
-    };
-    mark(i, j, k);
-    mark(i - 1, j, k); mark(i + 1, j, k);
-    mark(i, j - 1, k); mark(i, j + 1, k);
-    mark(i, j, k - 1); mark(i, j, k + 1);
+    static_assert(CW_BX - 2 + 2 <= 32 && CW_BY - 2 + 2 <= 16, "one thread column per box column, two row passes");
+    const int col = blockIdx.x % (ntx * nty), tz0 = (blockIdx.x / (ntx * nty)) * TM_ZT;
+    const int x0 = (col % ntx) * (CW_BX - 2), y0 = (col / ntx) * (CW_BY - 2);
+    const int bx = min(CW_BX - 2, nx - x0) + 2, by = min(CW_BY - 2, ny - y0) + 2;
+    const int lx = threadIdx.x & 31;
+    int x = x0 - 1 + lx;
+    x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+    const int ex = (lx == 0 || lx == bx - 1);
+    for (int tz = tz0; tz < min(tz0 + TM_ZT, ntz); ++tz) {
+        const int z0 = tz * zc, bz = min(zc, nz - z0) + 2;
+        int any = 0;
+        if (lx < bx) {
+            for (int ly = threadIdx.x >> 5; ly < by; ly += 8) {
+                int y = y0 - 1 + ly;
+                y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+                const int exy = ex + (ly == 0 || ly == by - 1);
+                if (exy > 1) continue;
+                for (int lz = 0; lz < bz; ++lz) {
+                    if (exy + (lz == 0 || lz == bz - 1) > 1) continue;
+                    int z = z0 - 1 + lz;
+                    z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+                    any |= flags[((size_t)z * ny + y) * nx + x];
+                }
+            }
+        }
+        any = __syncthreads_or(any);
+        if (threadIdx.x == 0) mask[((size_t)tz * nty) * ntx + col] = any ? 1 : 0;
+    }
 }
 
 size_t cweno_sel_tiles(const CwArgs &a)
@@ -428,10 +449,9 @@ size_t cweno_sel_tiles(const CwArgs &a)
 int launch_cweno_tilemask(const CwArgs &a, const uint8_t *flags, uint8_t *mask, hipStream_t s)
 {
     const int ntx = (a.nx + CW_BX - 3) / (CW_BX - 2), nty = (a.ny + CW_BY - 3) / (CW_BY - 2);
-    const size_t nt = cweno_sel_tiles(a), n = (size_t)a.nx * a.ny * a.nz;
-    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, mask, nt, (uint8_t)0);
-    hipLaunchKernelGGL(k_tile_mark, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, flags, a.nx, a.ny, a.nz, a.zc,
-                       ntx, nty, mask);
+    const int ntz = (a.nz + a.zc - 1) / a.zc;
+    hipLaunchKernelGGL(k_tile_mark, dim3((unsigned)((size_t)ntx * nty * ((ntz + TM_ZT - 1) / TM_ZT))), dim3(256), 0, s,
+                       flags, a.nx, a.ny, a.nz, a.zc, ntx, nty, ntz, mask);
     return hip_check(hipGetLastError(), "tile mask launch");
 }
 

@@ -32,14 +32,12 @@ __host__ __device__ inline int kperm(int g, int j) { return j < 4 ? 4 * g + j : 
 
 __device__ __forceinline__ float bf16_round(float x) { return (float)(__bf16)x; }
 
-// tanh(x) = sign(x) (1 - e) / (1 + e), e = exp(-2|x|) <= 1: one exponential
-// and one reciprocal (absolute error ~1e-7, far below the bf16 rounding the
-// layers apply to their operands).
-__device__ __forceinline__ float act_tanh(float x)
-{
-    const float e = __expf(-2.f * fabsf(x));
-    return copysignf((1.f - e) * __builtin_amdgcn_rcpf(1.f + e), x);
-}
+// Hidden activation: ReLU.  One VALU max per unit; an exponential-based
+// activation (tanh) made the fused inference pass bound by its transcendental
+// issue rate (two quarter-rate instructions per unit, 64 units per sample).
+// max(x, 0) as one integer max on the bit pattern: negative floats (and -0)
+// have negative int patterns; fmaxf would add a NaN-quieting step.
+__device__ __forceinline__ float act_relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 
 // ---- features --------------------------------------------------------------
 
@@ -136,39 +134,47 @@ __global__ void __launch_bounds__(256) k_labels(const double *__restrict__ U, Cw
 
 // ---- MLP forward on MFMA ------------------------------------------------------
 
+// Logit z of 16 samples per wavefront (y = sigmoid(z)); the biases enter as
+// the initial accumulators of the MFMAs (accumulator row 4g + i of lane l).
 __device__ __forceinline__ float mlp_wave(const MlpPacked &P, const bf16x8 &x, int lane, float (*h1out)[8],
                                           float (*h2out)[8])
 {
     const int g = lane >> 4;
-    floatx4 z = {0.f, 0.f, 0.f, 0.f};
-    floatx4 a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.w1[0][lane], x, z, 0, 0, 0);
-    floatx4 a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.w1[1][lane], x, z, 0, 0, 0);
-    float h[8];
+    floatx4 i0, i1;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        h[i] = act_tanh(a0[i] + P.b1[4 * g + i]);
-        h[4 + i] = act_tanh(a1[i] + P.b1[16 + 4 * g + i]);
+        i0[i] = P.b1[4 * g + i];
+        i1[i] = P.b1[16 + 4 * g + i];
     }
+    const floatx4 a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.w1[0][lane], x, i0, 0, 0, 0);
+    const floatx4 a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.w1[1][lane], x, i1, 0, 0, 0);
     bf16x8 hb;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) hb[q] = (__bf16)h[q];
+    for (int i = 0; i < 4; ++i) {
+        hb[i] = (__bf16)act_relu(a0[i]);
+        hb[4 + i] = (__bf16)act_relu(a1[i]);
+    }
     if (h1out) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) (*h1out)[q] = (float)hb[q];
     }
-    floatx4 c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.w2[0][lane], hb, z, 0, 0, 0);
-    floatx4 c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.w2[1][lane], hb, z, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        i0[i] = P.b2[4 * g + i];
+        i1[i] = P.b2[16 + 4 * g + i];
+    }
+    const floatx4 c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.w2[0][lane], hb, i0, 0, 0, 0);
+    const floatx4 c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.w2[1][lane], hb, i1, 0, 0, 0);
     float acc = 0.f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const float u0 = act_tanh(c0[i] + P.b2[4 * g + i]);
-        const float u1 = act_tanh(c1[i] + P.b2[16 + 4 * g + i]);
+        const float u0 = act_relu(c0[i]), u1 = act_relu(c1[i]);
         if (h2out) { (*h2out)[i] = u0; (*h2out)[4 + i] = u1; }
         acc += P.w3[4 * g + i] * u0 + P.w3[16 + 4 * g + i] * u1;
     }
     acc += __shfl_xor(acc, 16, 64);
     acc += __shfl_xor(acc, 32, 64);
-    return 1.f / (1.f + __expf(-(acc + P.b3)));
+    return acc + P.b3;
 }
 
 __global__ void __launch_bounds__(256) k_mlp_forward(const MlpPacked *__restrict__ Pg, const float *__restrict__ X,
@@ -187,7 +193,7 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const MlpPacked *__restrict
 #pragma unroll
         for (int q = 0; q < 8; ++q) x[q] = (__bf16)(s < n ? X[(size_t)s * ACX_MLP_IN + 8 * g + q] : 0.f);
         float h1[8], h2[8];
-        const float y = mlp_wave(P, x, lane, H1 ? &h1 : nullptr, H2 ? &h2 : nullptr);
+        const float y = 1.f / (1.f + __expf(-mlp_wave(P, x, lane, H1 ? &h1 : nullptr, H2 ? &h2 : nullptr)));
         if (s < n) {
             if (g == 0) Y[s] = y;
 #pragma unroll
@@ -203,32 +209,41 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const MlpPacked *__restrict
 
 constexpr int IND_WAVES = 4;
 
-// Each wave pass covers 16 cells x 4 components = 64 samples: lane l forms
-// the features of sample l (cell c0 + l/4, component l%4; the 64 lanes read
-// 64 consecutive floats per stencil point), parks them in LDS, and the wave
-// then runs four 16-sample MFMA batches over them.
+// Each wave pass covers 16 cells of an x-row x 4 components = 64 samples:
+// lane l forms the features of sample l (cell i0 + l/4 of the row, component
+// l%4; the 64 lanes read 64 consecutive floats per stencil point), parks them
+// in LDS, and the wave then runs four 16-sample MFMA batches over them.
 __global__ void __launch_bounds__(64 * IND_WAVES) k_indicator(const MlpPacked *__restrict__ Pg,
                                                              const float *__restrict__ HI,
                                                              const float *__restrict__ D, int nx, int ny, int nz,
-                                                             float thr, uint8_t *__restrict__ flags)
+                                                             float zthr, uint8_t *__restrict__ flags)
 {
     __shared__ MlpPacked P;
-    __shared__ float feat[IND_WAVES][64][ACX_MLP_IN + 1];
+    // features parked as the bf16 operands the MFMA takes (rows of 40: the
+    // 16-byte K slices stay aligned); half the LDS of fp32 rows, so more
+    // waves fit per CU
+    constexpr int FROW = ACX_MLP_IN + 8;
+    __shared__ __align__(16) __bf16 feat[IND_WAVES][64][FROW];
     for (int q = threadIdx.x; q < (int)(sizeof(MlpPacked) / 4); q += blockDim.x)
         reinterpret_cast<float *>(&P)[q] = reinterpret_cast<const float *>(Pg)[q];
     __syncthreads();
-    const size_t ncell = (size_t)nx * ny * nz;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4, sl = lane & 15;
-    float(*F)[ACX_MLP_IN + 1] = feat[wv];
-    const size_t nw = (size_t)gridDim.x * IND_WAVES;
-    for (size_t c0 = ((size_t)blockIdx.x * IND_WAVES + wv) * 16; c0 < ncell; c0 += nw * 16) {
+    // one wave pass = 16 cells of one x-row (segment seg of row (j, k)) x 4
+    // components; the pass index and its row are wave-uniform (scalar math)
+    const int lane = threadIdx.x & 63, g = lane >> 4, sl = lane & 15;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    __bf16(*F)[FROW] = feat[wv];
+    const unsigned segs = (unsigned)(nx + 15) / 16u;
+    const unsigned npass = segs * (unsigned)ny * (unsigned)nz;
+    const unsigned nw = gridDim.x * IND_WAVES;
+    for (unsigned ps = blockIdx.x * IND_WAVES + (unsigned)wv; ps < npass; ps += nw) {
+        const unsigned row = ps / segs;
+        const int i0 = (int)(ps - row * segs) * 16;
+        const int j = (int)(row % (unsigned)ny), k = (int)(row / (unsigned)ny);
         {
-            const unsigned cell = (unsigned)c0 + (unsigned)(lane >> 2);
+            const int i = i0 + (lane >> 2);
             const int v = lane & 3;
             float f[ACX_MLP_NFEAT];
-            if (cell < ncell) {
-                const unsigned row = cell / (unsigned)nx;
-                const int i = (int)(cell - row * (unsigned)nx), j = (int)(row % (unsigned)ny), k = (int)(row / (unsigned)ny);
+            if (i < nx) {
                 float a[3][5];
                 gather_arms(HI, D, nx, ny, nz, i, j, k, v, a);
                 features19(a, f);
@@ -237,20 +252,18 @@ __global__ void __launch_bounds__(64 * IND_WAVES) k_indicator(const MlpPacked *_
                 for (int q = 0; q < ACX_MLP_NFEAT; ++q) f[q] = 0.f;
             }
 #pragma unroll
-            for (int q = 0; q < ACX_MLP_IN; ++q) F[lane][q] = q < ACX_MLP_NFEAT ? f[q] : 0.f;
+            for (int q = 0; q < ACX_MLP_IN; ++q) F[lane][q] = (__bf16)(q < ACX_MLP_NFEAT ? f[q] : 0.f);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            bf16x8 x;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) x[q] = (__bf16)F[16 * b + sl][8 * g + q];
-            const float y = mlp_wave(P, x, lane, nullptr, nullptr);
-            const unsigned long long m = __ballot(y > thr);   // bit n (< 16): sample 16b + n
-            const size_t cell = c0 + 4 * b + lane;
-            if (lane < 4 && cell < ncell) flags[cell] = (uint8_t)((m >> (4 * lane)) & 0xF);
+            const bf16x8 x = *reinterpret_cast<const bf16x8 *>(&F[16 * b + sl][8 * g]);
+            const float z = mlp_wave(P, x, lane, nullptr, nullptr);
+            const unsigned long long m = __ballot(z > zthr);   // bit n (< 16): sample 16b + n
+            const int ic = i0 + 4 * b + lane;
+            if (lane < 4 && ic < nx) flags[(size_t)row * nx + ic] = (uint8_t)((m >> (4 * lane)) & 0xF);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();   // every read of F is done before the next pass writes it
@@ -294,7 +307,7 @@ __global__ void __launch_bounds__(256) k_mlp_backward(const float *__restrict__ 
     for (int q = threadIdx.x; q < BW_SAMPLES * 32; q += blockDim.x) {
         const int a = q / 32, j = q % 32;
         const float h = h2s[a][j];
-        dh2[a][j] = dzs[a] * w3[j] * (1.f - h * h);
+        dh2[a][j] = h > 0.f ? dzs[a] * w3[j] : 0.f;
     }
     __syncthreads();
     for (int q = threadIdx.x; q < BW_SAMPLES * 32; q += blockDim.x) {
@@ -303,7 +316,7 @@ __global__ void __launch_bounds__(256) k_mlp_backward(const float *__restrict__ 
 #pragma unroll 8
         for (int j = 0; j < 32; ++j) acc += dh2[a][j] * sW2[j][kk];
         const float h = h1s[a][kk];
-        dh1[a][kk] = acc * (1.f - h * h);
+        dh1[a][kk] = h > 0.f ? acc : 0.f;
     }
     __syncthreads();
     // parameter gradients: sum over the block's samples, one atomic per entry
@@ -376,17 +389,28 @@ __global__ void k_pack(const float *__restrict__ prm, MlpPacked *__restrict__ P)
     if (t == 0) P->b3 = b3[0];
 }
 
-__global__ void k_dilate(const uint8_t *__restrict__ in, uint8_t *__restrict__ out, int nx, int ny, int nz)
+// Block = 64 x 4 columns marching a chunk of DIL_ZC planes; the own column
+// of planes k-1, k, k+1 stays in registers (no per-cell index division).
+constexpr int DIL_BX = 64, DIL_BY = 4, DIL_ZC = 16;
+
+__global__ void __launch_bounds__(DIL_BX *DIL_BY) k_dilate(const uint8_t *__restrict__ in, uint8_t *__restrict__ out,
+                                                          int nx, int ny, int nz)
 {
-    const size_t ncell = (size_t)nx * ny * nz;
-    const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((size_t)nx * ny));
-    auto W = [](int q, int n) { return q < 0 ? q + n : (q >= n ? q - n : q); };
-    auto at = [&](int x, int y, int z) { return in[((size_t)W(z, nz) * ny + W(y, ny)) * nx + W(x, nx)]; };
-    const uint8_t any = at(i, j, k) | at(i - 1, j, k) | at(i + 1, j, k) | at(i, j - 1, k) | at(i, j + 1, k) |
-                        at(i, j, k - 1) | at(i, j, k + 1);
-    out[c] = any ? 0xF : 0;   // the four acoustic components move together
+    const int i = blockIdx.x * DIL_BX + threadIdx.x, j = blockIdx.y * DIL_BY + threadIdx.y;
+    if (i >= nx || j >= ny) return;
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+    const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+    const size_t plane = (size_t)nx * ny;
+    const int z0 = blockIdx.z * DIL_ZC, z1 = min(z0 + DIL_ZC, nz);
+    auto at = [&](int k, int y, int x) { return in[(size_t)k * plane + (size_t)y * nx + x]; };
+    uint8_t below = at(z0 == 0 ? nz - 1 : z0 - 1, j, i), cur = at(z0, j, i);
+    for (int k = z0; k < z1; ++k) {
+        const uint8_t above = at(k == nz - 1 ? 0 : k + 1, j, i);
+        const uint8_t any = below | cur | above | at(k, j, im) | at(k, j, ip) | at(k, jm, i) | at(k, jp, i);
+        out[(size_t)k * plane + (size_t)j * nx + i] = any ? 0xF : 0;   // the four acoustic components move together
+        below = cur;
+        cur = above;
+    }
 }
 
 __global__ void k_count(const uint8_t *__restrict__ f, size_t n, unsigned long long *__restrict__ out)
@@ -493,22 +517,23 @@ int acx_indicator_flags(const acx_grid *g, const float *HI, const float *D, cons
     int rc = check_ac4(g);
     if (rc) return rc;
     if (!HI || !packed || !flags) return set_error(ACX_EINVAL, "indicator: null pointer");
-    const size_t nc = (size_t)g->n[0] * g->n[1] * g->n[2];
-    const size_t groups = (nc + 15) / 16;   // one wave pass per 16 cells
+    // sigmoid(z) > thr  <=>  z > log(thr / (1 - thr)) for 0 < thr < 1
+    const float zthr = !(thr > 0.f) ? -INFINITY : (thr >= 1.f ? INFINITY : logf(thr / (1.f - thr)));
+    const size_t groups = (size_t)((g->n[0] + 15) / 16) * g->n[1] * g->n[2];   // one wave pass per 16 cells of a row
     size_t blocks = (groups + IND_WAVES - 1) / IND_WAVES;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_indicator, dim3((unsigned)blocks), dim3(64 * IND_WAVES), 0, (hipStream_t)stream,
                        (const MlpPacked *)packed,
-                       HI, D, g->n[0], g->n[1], g->n[2], thr, flags);
+                       HI, D, g->n[0], g->n[1], g->n[2], zthr, flags);
     return hip_check(hipGetLastError(), "indicator");
 }
 
 int acx_flags_dilate(const acx_grid *g, const uint8_t *in, uint8_t *out, void *stream)
 {
     if (!g || !in || !out || in == out) return set_error(ACX_EINVAL, "dilate: bad arguments");
-    const size_t nc = (size_t)g->n[0] * g->n[1] * g->n[2];
-    hipLaunchKernelGGL(k_dilate, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, (hipStream_t)stream, in, out, g->n[0],
-                       g->n[1], g->n[2]);
+    const dim3 grd((unsigned)((g->n[0] + DIL_BX - 1) / DIL_BX), (unsigned)((g->n[1] + DIL_BY - 1) / DIL_BY),
+                   (unsigned)((g->n[2] + DIL_ZC - 1) / DIL_ZC));
+    hipLaunchKernelGGL(k_dilate, grd, dim3(DIL_BX, DIL_BY), 0, (hipStream_t)stream, in, out, g->n[0], g->n[1], g->n[2]);
     return hip_check(hipGetLastError(), "dilate");
 }
 
@@ -524,7 +549,7 @@ int acx_flags_count(const uint8_t *flags, size_t ncells, long long *bits, void *
     unsigned long long h = 0;
     rc = hip_check(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s), "count copy");
     if (!rc) rc = hip_check(hipStreamSynchronize(s), "count sync");
-    hipFreeAsync(d, s);
+    (void)hipFreeAsync(d, s);
     *bits = (long long)h;
     return rc;
 }

@@ -15,7 +15,7 @@ import math
 
 import numpy as np
 
-from . import _check, _ptr, _stream, lib, ACOUSTIC4, Grid, CompensatedState, field_noise
+from . import _check, _ptr, _stream, lib, ACOUSTIC4, Grid, CompensatedState, field_noise, sel_mask
 
 NPARAM = 32 * 32 + 32 + 32 * 32 + 32 + 32 + 1
 NFEAT = 19
@@ -53,14 +53,14 @@ def _g(grid):
 
 
 def init_params(seed=0):
-    """Xavier-uniform weights, zero biases (fp32 numpy, NPARAM)."""
+    """He-uniform weights (ReLU layers), zero biases (fp32 numpy, NPARAM)."""
     rng = np.random.default_rng(seed)
     p = np.zeros(NPARAM, np.float32)
-    lim1 = math.sqrt(6.0 / (NFEAT + 32))
+    lim1 = math.sqrt(6.0 / NFEAT)
     w1 = rng.uniform(-lim1, lim1, (32, 32)).astype(np.float32)
     w1[:, NFEAT:] = 0.0
     p[OFF["w1"]:OFF["w1"] + 1024] = w1.ravel()
-    lim2 = math.sqrt(6.0 / 64)
+    lim2 = math.sqrt(6.0 / 32)
     p[OFF["w2"]:OFF["w2"] + 1024] = rng.uniform(-lim2, lim2, 1024)
     p[OFF["w3"]:OFF["w3"] + 32] = rng.uniform(-lim2, lim2, 32)
     return p
@@ -232,6 +232,7 @@ class SelectiveStepper:
         self.state = CompensatedState(grid, U64)
         self.flags = torch.empty(grid.ncells, dtype=torch.uint8, device="cuda")
         self.tmp = torch.empty_like(self.flags)
+        self.mask = None
         self.nsteps = 0
         self.flag_bits = []
 
@@ -248,11 +249,13 @@ class SelectiveStepper:
             src, dst = dst, src
         if src is not self.flags:
             self.flags.copy_(src)
+        # the patch tiles of these flags, kept until the next refresh
+        self.mask = sel_mask(self.grid, self.flags, self.mask, stream)
 
     def step(self, dt):
         if self.nsteps % self.refresh == 0:
             self.refresh_flags()
-        self.state.step(dt, flags=self.flags)
+        self.state.step(dt, flags=self.flags, mask=self.mask)
         self.nsteps += 1
 
     def flagged_fraction(self):

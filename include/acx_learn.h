@@ -12,7 +12,7 @@
  * 4th-order polynomial while full CWENO stays the fallback everywhere else.
  *
  * Network: 19 inputs (18 normalised undivided differences, 6 per axis, plus
- * a log-amplitude term), padded to 32 -> 32 tanh -> 32 tanh -> 1 sigmoid.
+ * a log-amplitude term), padded to 32 -> 32 ReLU -> 32 ReLU -> 1 sigmoid.
  * Inference and the training forward run on MFMA (bf16 operands, fp32
  * accumulation); the backward pass and Adam run in fp32.
  */
@@ -81,12 +81,27 @@ int acx_flags_count(const uint8_t *flags, size_t ncells, long long *bits, void *
 /* One compensated FP32 SSP-RK3 step with the selective reconstruction:
  * full CWENO4 (grid gammas) where the flag bit of a component is set, the
  * linear 4th-order polynomial elsewhere.  Each stage runs the linear-scheme
- * kernel over every cell, then the CWENO kernel over the tiles (30 x 14 x 16
- * cells) that hold a flagged cell or a face neighbour of one.  The tile mask
- * is library scratch kept per device: selective steps on one device must not
- * run concurrently.  ACOUSTIC4, periodic. */
+ * kernel over every cell, then the CWENO kernel over the tiles (30 x 14 x 4
+ * cells; ACX_ZC_PATCH sets the 4, read once per process) that hold a flagged
+ * cell or a face neighbour of one.  The tile mask is made from the flags on
+ * every call, into library scratch kept per device: selective steps on one
+ * device must not run concurrently.  ACOUSTIC4. */
 int acx_ssp_rks3_step_f32c_sel(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
                                float *LO_OUT, float *D0, float *D1, const uint8_t *flags, double dt, void *stream);
+
+/* The tile mask of the selective step for these flags: acx_sel_mask_bytes(g)
+ * bytes (0 for an invalid grid), one per patch tile, 1 where the tile holds a
+ * flagged cell or a face neighbour of one. */
+size_t acx_sel_mask_bytes(const acx_grid *g);
+int acx_sel_mask(const acx_grid *g, const uint8_t *flags, uint8_t *mask, void *stream);
+
+/* acx_ssp_rks3_step_f32c_sel with the tile mask made by acx_sel_mask from the
+ * same flags (flags change only when the indicator runs, so a caller that
+ * keeps flags over several steps makes the mask once).  No library scratch:
+ * steps on different states may run concurrently. */
+int acx_ssp_rks3_step_f32c_selm(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
+                                float *LO_OUT, float *D0, float *D1, const uint8_t *flags, const uint8_t *mask,
+                                double dt, void *stream);
 
 #ifdef __cplusplus
 }

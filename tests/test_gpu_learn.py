@@ -25,9 +25,9 @@ def cpu_forward(p, X):
     W1, b1, W2, b2, w3, b3 = split(p)
     W1r, W2r = bf16(W1).astype(np.float64), bf16(W2).astype(np.float64)
     x = bf16(X).astype(np.float64)
-    h1 = np.tanh(x @ W1r.T + b1)
+    h1 = np.maximum(x @ W1r.T + b1, 0.0)
     h1r = bf16(h1).astype(np.float64)
-    h2 = np.tanh(h1r @ W2r.T + b2)
+    h2 = np.maximum(h1r @ W2r.T + b2, 0.0)
     y = 1.0 / (1.0 + np.exp(-(h2 @ w3 + b3)))
     return y, h1r, h2
 
@@ -49,7 +49,7 @@ def test_mfma_forward_matches_cpu(acx, torch_gpu):
     Y, H1, H2 = learn.forward(packed, torch.from_numpy(X).cuda(), keep=True)
     y, h1r, h2 = cpu_forward(p, X)
     assert np.max(np.abs(Y.cpu().numpy() - y)) < 2e-5
-    assert np.max(np.abs(H1.cpu().numpy() - h1r)) < 1e-2   # bf16 ulp where tanh rounds differently
+    assert np.max(np.abs(H1.cpu().numpy() - h1r)) < 1e-2   # bf16 ulp where the fp32 sum rounds differently
     assert np.max(np.abs(H2.cpu().numpy() - h2)) < 2e-5
 
 
@@ -69,35 +69,48 @@ def test_backward_matches_cpu(acx, torch_gpu):
     pw = 3.0
     G = learn.backward(pd, Xd, H1, H2, Y, torch.from_numpy(T).cuda(), pw).cpu().numpy().astype(np.float64)
     # CPU FP64 gradient on the GPU's forward quantities
-    W1, b1, W2, b2, w3, b3 = split(p)
     y, h1r, h2 = Y.cpu().numpy().astype(np.float64), H1.cpu().numpy().astype(np.float64), H2.cpu().numpy().astype(np.float64)
     t = T.astype(np.float64)
     c = np.where(T > 0, pw, 1.0)
-    dz = c * (y - t) / n
-    dh2 = dz[:, None] * w3[None, :] * (1 - h2 ** 2)
-    dh1 = (dh2 @ bf16(W2).astype(np.float64)) * (1 - h1r ** 2)
-    xr = bf16(X).astype(np.float64)
-    ref = np.concatenate([(dh1.T @ xr).ravel(), dh1.sum(0), (dh2.T @ h1r).ravel(), dh2.sum(0), (dz[:, None] * h2).sum(0),
-                          [dz.sum()]])
+    ref = cpu_grad(p, bf16(X).astype(np.float64), bf16(split(p)[2]).astype(np.float64), y, h1r, h2, c, t)
     err = np.linalg.norm(G - ref) / np.linalg.norm(ref)
     print(f"[case] backward rel err {err:.2e}", flush=True)
     assert err < 1e-5
-    # and a central-difference check of the FP64 loss (no bf16 rounding: the
-    # bf16 grid is coarser than the step) for weights of every layer; the
-    # gradient of the bf16 forward differs from it by the operand rounding
-    def loss(pp):
+    # the gradient formula itself against central differences of the FP64
+    # loss (FP64 network, no bf16 rounding), for weights of every layer
+    W1, b1, W2, b2, w3, b3 = split(p)
+    x = X.astype(np.float64)
+
+    def fwd(pp):
         W1, b1, W2, b2, w3, b3 = split(pp)
-        x = X.astype(np.float64)
-        h2 = np.tanh(np.tanh(x @ W1.T + b1) @ W2.T + b2)
-        yy = np.clip(1.0 / (1.0 + np.exp(-(h2 @ w3 + b3))), 1e-12, 1 - 1e-12)
+        h1 = np.maximum(x @ W1.T + b1, 0.0)
+        h2 = np.maximum(h1 @ W2.T + b2, 0.0)
+        return 1.0 / (1.0 + np.exp(-(h2 @ w3 + b3))), h1, h2
+
+    def loss(pp):
+        yy = np.clip(fwd(pp)[0], 1e-12, 1 - 1e-12)
         return -np.sum(c * (t * np.log(yy) + (1 - t) * np.log(1 - yy))) / n
+    y64, h164, h264 = fwd(p.astype(np.float64))
+    g64 = cpu_grad(p, x, W2, y64, h164, h264, c, t)
     for k in (2144, 2120, 2090, 1100, 1060, 1030, 5, 300):
-        e = 1e-4
+        e = 1e-6
         pp, pm = p.astype(np.float64).copy(), p.astype(np.float64).copy()
         pp[k] += e
         pm[k] -= e
         fd = (loss(pp) - loss(pm)) / (2 * e)
-        assert abs(fd - G[k]) < 0.03 * abs(fd) + 3e-4 * np.abs(G).max(), (k, fd, G[k])
+        assert abs(fd - g64[k]) < 1e-5 * abs(fd) + 1e-6 * np.abs(g64).max(), (k, fd, g64[k])
+
+
+def cpu_grad(p, x, W2, y, h1, h2, c, t):
+    """Gradient of the class-weighted cross entropy of the ReLU MLP, from its
+    forward quantities (inputs x, layer-2 weights W2 as the forward used them)."""
+    n = x.shape[0]
+    w3 = split(p)[4]
+    dz = c * (y - t) / n
+    dh2 = dz[:, None] * w3[None, :] * (h2 > 0)
+    dh1 = (dh2 @ W2) * (h1 > 0)
+    return np.concatenate([(dh1.T @ x).ravel(), dh1.sum(0), (dh2.T @ h1).ravel(), dh2.sum(0), (dz[:, None] * h2).sum(0),
+                           [dz.sum()]])
 
 
 def test_labels_match_oracle_weights(acx, orc, torch_gpu):
@@ -258,3 +271,39 @@ def test_tile_patch_equals_full_selective_kernel(acx, torch_gpu, monkeypatch):
     err = np.linalg.norm(ra - rb) / np.linalg.norm(rb)
     print(f"[case] tile patch vs full selective kernel: {err:.2e}", flush=True)
     assert err < 1e-6
+
+
+def test_tile_mask_and_masked_step(acx, torch_gpu):
+    """acx_sel_mask marks a patch tile (30 x 14 x zc) exactly when it holds a
+    flagged cell or a periodic face neighbour of one; the step with that mask
+    is bitwise the step that makes the mask itself."""
+    torch = torch_gpu
+    from acoustic_petsc_amd import learn
+    g = acx.Grid(n=(47, 33, 21), h=1.0 / 47, gamma_hi=0.85, gamma_lo=0.85)
+    rng = np.random.default_rng(12)
+    fl = np.zeros(g.ncells, np.uint8)
+    idx = rng.choice(g.ncells, 40, replace=False)
+    fl[idx] = rng.integers(1, 16, idx.size)
+    flags = torch.from_numpy(fl).cuda()
+    mask = acx.sel_mask(g, flags).cpu().numpy()
+    nx, ny, nz = g.n
+    ntx, nty = -(-nx // 30), -(-ny // 14)
+    assert mask.size % (ntx * nty) == 0
+    ntz = mask.size // (ntx * nty)
+    zc = next(c for c in range(1, nz + 1) if -(-nz // c) == ntz)   # the tiling's z-chunk
+    want = np.zeros(mask.size, np.uint8)
+    f3 = fl.reshape(nz, ny, nx)
+    for k, j, i in zip(*np.nonzero(f3)):
+        for dx, dy, dz in ((0, 0, 0), (-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)):
+            x, y, z = (i + dx) % nx, (j + dy) % ny, (k + dz) % nz
+            want[((z // zc) * nty + y // 14) * ntx + x // 30] = 1
+    print(f"[case] tile mask {mask.size} tiles, zc {zc}, marked {int(mask.sum())}", flush=True)
+    assert np.array_equal(mask, want)
+    U = torch.from_numpy(learn.steep_pulse_field(g, 5)).cuda()
+    dt = acx.cfl_dt(g, 0.4)
+    a, b = acx.CompensatedState(g, U), acx.CompensatedState(g, U)
+    mt = torch.from_numpy(mask).cuda()
+    for _ in range(2):
+        a.step(dt, flags=flags)
+        b.step(dt, flags=flags, mask=mt)
+    assert torch.equal(a.hi, b.hi) and torch.equal(a.lo, b.lo)
