@@ -61,8 +61,8 @@ static int launch_linear_v(const LinArgs &a, hipStream_t s)
 // 64 x 8 with late epilogue loads (10), and capped for 4 waves per SIMD (11);
 // centre-plane 16 x 32 (12); centre-plane 32 x 16 with streaming epilogue
 // loads and stores (13); centre-plane 32 x 16 with the halo ring (14), and
-// with streaming epilogue (15); halo ring with loads 2 (16) or 3 (17)
-// planes ahead.
+// with streaming epilogue (15); any other value: lead-plane 64 x 8 with loads
+// 2 planes ahead.
 // Defaults (measured at 256^3 and 384^3): FP64 variant 14, FP32 variant 0.
 // Grids with a wall on any axis take the default variant of their precision
 // instantiated with ghost signs.
