@@ -16,7 +16,11 @@ struct LinTile {
 };
 
 // Epilogue kinds
-enum { EPI_RHS = 0, EPI_AXPY = 1, EPI_COMB = 2, EPI_F32C = 3 };
+// EPI_F32CF: the compensated FP32 stage that ends the step (writes the new
+// HI and LO; p.final_stage is implied), instantiated on its own so that the
+// lead-plane march can keep HI of the last four planes in registers for the
+// epilogue's exact sum instead of loading it again three planes later.
+enum { EPI_RHS = 0, EPI_AXPY = 1, EPI_COMB = 2, EPI_F32C = 3, EPI_F32CF = 4 };
 
 // One tile: columns [x0, x0+BX) x [y0, y0+BY), planes [zb, ze) of the grid
 // described by p (shared by the per-grid kernel and the ensemble kernel).
@@ -90,8 +94,9 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
     const T inv48h = T(p.inv48h), Kc = T(p.K), irho = T(p.invrho), sLLF = T(p.s);
     const T lap = T(p.lapc);   // WAVE2: c^2 / (48 h^2)
     const int jbeg = zb - 3, jend = ze + 2;
-    const bool has_d = (EPI == EPI_F32C) && D != nullptr;
-    const bool fin_stage = (EPI == EPI_F32C) && p.final_stage;
+    constexpr bool F32C = EPI == EPI_F32C || EPI == EPI_F32CF;
+    const bool has_d = F32C && D != nullptr;
+    const bool fin_stage = EPI == EPI_F32CF || (EPI == EPI_F32C && p.final_stage);
 
     struct Slot {
         Cell<T, NV> q, d, hq, hd, e0, e1, e2;
@@ -112,10 +117,11 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
         if (c >= zb && c < ze && active) {
             const size_t cidx = (size_t)c * plane + col;
             if constexpr (EPI == EPI_COMB) S.e0 = load_cell<T, NV>(U0, cidx);
-            if constexpr (EPI == EPI_F32C) {
+            if constexpr (F32C) {
                 if (fin_stage) {
                     S.e1 = load_cell<T, NV>(LO, cidx);
-                    S.e2 = load_cell<T, NV>(Q, cidx);   // HI of plane c, exact
+                    // HI of plane c, exact (EPI_F32CF: from the register queue)
+                    if constexpr (EPI == EPI_F32C) S.e2 = load_cell<T, NV>(Q, cidx);
                 }
             }
         }
@@ -125,11 +131,17 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
     T xyq[4][NP];        // in-plane RHS parts of planes j-3 .. j
     // F32C: the increment D of planes j-3 .. j, kept from the stencil loads for
     // the epilogue of plane j-3 (a second load of it three planes later missed L2)
-    Cell<T, NV> dq[EPI == EPI_F32C ? 4 : 1];
+    Cell<T, NV> dq[F32C ? 4 : 1];
+    // EPI_F32CF: HI of planes j-3 .. j as loaded (before adding D)
+    Cell<T, NV> hq[EPI == EPI_F32CF ? 4 : 1];
 #pragma unroll
-    for (int k = 0; k < (EPI == EPI_F32C ? 4 : 1); ++k)
+    for (int k = 0; k < (F32C ? 4 : 1); ++k)
 #pragma unroll
         for (int v = 0; v < NV; ++v) dq[k].v[v] = T(0);
+#pragma unroll
+    for (int k = 0; k < (EPI == EPI_F32CF ? 4 : 1); ++k)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) hq[k].v[v] = T(0);
 #pragma unroll
     for (int k = 0; k < 7; ++k)
 #pragma unroll
@@ -155,10 +167,15 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
 
             // consume the slot of plane j
             Cell<T, NV> cur = S.q;
+            if constexpr (EPI == EPI_F32CF) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) hq[k] = hq[k + 1];
+                hq[3] = S.q;
+            }
             if (has_d) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) cur.v[v] += S.d.v[v];
-                if constexpr (EPI == EPI_F32C) {
+                if constexpr (F32C) {
 #pragma unroll
                     for (int k = 0; k < 3; ++k) dq[k] = dq[k + 1];
                     dq[3] = S.d;
@@ -271,7 +288,7 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
                         Cell<T, NV> nh, nl;
 #pragma unroll
                         for (int v = 0; v < NV; ++v) {
-                            const float hi = (float)S.e2.v[v];
+                            const float hi = (float)(EPI == EPI_F32CF ? hq[0].v[v] : S.e2.v[v]);
                             const float t = (float)S.e1.v[v] + (float)o.v[v];
                             float sm, er;
                             two_sum(hi, t, sm, er);
@@ -328,7 +345,9 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
     const bool active = (x0 + tx < nx) && (y0 + ty < ny);
     const size_t plane = (size_t)nx * ny;
     const int bcx = WALLS ? p.bc[0] : 0, bcy = WALLS ? p.bc[1] : 0, bcz = WALLS ? p.bc[2] : 0;
-    const bool has_d = (EPI == EPI_F32C) && D != nullptr;
+    constexpr bool F32C = EPI == EPI_F32C || EPI == EPI_F32CF;
+    const bool has_d = F32C && D != nullptr;
+    const bool fin_stage = EPI == EPI_F32CF || (EPI == EPI_F32C && p.final_stage);
 
     float sgx = 1.f, sgy = 1.f;
     const int mx = map_axis(x0 + tx, nx, bcx, sgx);
@@ -425,9 +444,9 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
         Cell<T, NV> e0, e1, e2;
         auto epi_loads = [&]() {
             if constexpr (EPI == EPI_COMB) e0 = NT ? load_cell_nt<T, NV>(U0, idx) : load_cell<T, NV>(U0, idx);
-            if constexpr (EPI == EPI_F32C) {
+            if constexpr (F32C) {
                 if (has_d) e0 = load_cell<T, NV>(D, idx);
-                if (p.final_stage) {
+                if (fin_stage) {
                     e1 = load_cell<T, NV>(LO, idx);
                     e2 = load_cell<T, NV>(Q, idx);
                 }
@@ -501,7 +520,7 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
                 const T bb = T(p.b), cc = T(p.c);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) o.v[v] = (has_d ? bb * e0.v[v] : T(0)) + cc * r.v[v];
-                if (p.final_stage) {
+                if (fin_stage) {
                     Cell<T, NV> nh, nl;
 #pragma unroll
                     for (int v = 0; v < NV; ++v) {
