@@ -29,7 +29,8 @@ constexpr int ENS_MAXP = 256;
 struct EnsConfig { int by, zc; };
 static const EnsConfig kConfigs[6] = {{8, 16}, {8, 32}, {8, 64}, {16, 16}, {16, 32}, {16, 64}};
 
-template <int BY>
+// EPI: EPI_F32C for stages 1-3, EPI_F32CF for the stage that ends the step.
+template <int BY, int EPI>
 __global__ void __launch_bounds__(ENS_BX *BY)
 k_ensemble(const EnsProblem *__restrict__ probs, int nprob, int total, unsigned *__restrict__ counter, LinArgs base)
 {
@@ -62,7 +63,7 @@ k_ensemble(const EnsProblem *__restrict__ probs, int nprob, int total, unsigned 
         a.out2 = base.out2 ? static_cast<float *>(base.out2) + o : nullptr;
         const int x0 = (local % pr.ntx) * ENS_BX, y0 = ((local / pr.ntx) % pr.nty) * BY;
         const int zb = (local / (pr.ntx * pr.nty)) * a.zc;
-        linear_tile<float, ACX_ACOUSTIC4, ENS_BX, BY, EPI_F32C, 1, false>(a, x0, y0, zb, lds);   // loads one plane ahead, periodic
+        linear_tile<float, ACX_ACOUSTIC4, ENS_BX, BY, EPI, 1, false>(a, x0, y0, zb, lds);   // loads one plane ahead, periodic
         __syncthreads();   // LDS is reused by the next item
     }
 }
@@ -124,8 +125,12 @@ static int ens_launch(acx_ensemble *e, const LinArgs &a, hipStream_t s)
 {
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(1), 0, s, e->d_counter);
     const int nb = e->persistent_blocks[BY == 8 ? 0 : 1];
-    hipLaunchKernelGGL(k_ensemble<BY>, dim3(nb), dim3(ENS_BX, BY), 0, s, e->d_probs, e->nprob, ens_total(e),
-                       e->d_counter, a);
+    if (a.final_stage)
+        hipLaunchKernelGGL((k_ensemble<BY, EPI_F32CF>), dim3(nb), dim3(ENS_BX, BY), 0, s, e->d_probs, e->nprob,
+                           ens_total(e), e->d_counter, a);
+    else
+        hipLaunchKernelGGL((k_ensemble<BY, EPI_F32C>), dim3(nb), dim3(ENS_BX, BY), 0, s, e->d_probs, e->nprob,
+                           ens_total(e), e->d_counter, a);
     return hip_check(hipGetLastError(), "ensemble launch");
 }
 
@@ -196,11 +201,16 @@ int acx_ensemble_create(int nprob, const int32_t *n, double h, acx_ensemble **ou
         hipMemset(e->hi[0], 0, bytes);
         hipMemset(e->lo[0], 0, bytes);
         e->cur = 0;
-        int dev = 0, cus = 0, b8 = 0, b16 = 0;
+        int dev = 0, cus = 0, b8 = 0, b16 = 0, f8 = 0, f16 = 0;
         hipGetDevice(&dev);
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b8, k_ensemble<8>, ENS_BX * 8, 0);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b16, k_ensemble<16>, ENS_BX * 16, 0);
+        // persistent grid: blocks that fit at once, the fewer of the two epilogue kinds
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b8, k_ensemble<8, EPI_F32C>, ENS_BX * 8, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b16, k_ensemble<16, EPI_F32C>, ENS_BX * 16, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&f8, k_ensemble<8, EPI_F32CF>, ENS_BX * 8, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&f16, k_ensemble<16, EPI_F32CF>, ENS_BX * 16, 0);
+        b8 = f8 < b8 ? f8 : b8;
+        b16 = f16 < b16 ? f16 : b16;
         e->persistent_blocks[0] = cus * (b8 > 0 ? b8 : 1);
         e->persistent_blocks[1] = cus * (b16 > 0 ? b16 : 1);
         rc = ens_layout(e);
