@@ -88,29 +88,33 @@ def test_quadratic_data_gives_linear_weights(orc, gamma):
     assert np.max(np.abs(co - c)) < 1e-12
 
 
+@pytest.mark.parametrize("axis", ["x", "y", "z"])
 @pytest.mark.parametrize("scale", [1.0, 1e-6])
-def test_sided_weights_follow_the_indicator_ratio(orc, scale):
-    """A block that is one quadratic q1 on the x >= 0 side and another, q2 = q1
-    + b x + c (x^2 - 1/12), on the x <= 0 side (the two have the same cell
-    averages on the x = 0 plane).  The +x stencil reads only x >= 0 cells and
-    the -x stencil only x <= 0 cells, so each reproduces its quadratic and its
-    indicator is that quadratic's derivative energy (sympy, orders 1 and 2).
-    With the published weights gamma / (IS + eps)^4, eps = 1e-12
-    (reconstruction.c:205-213, hype.h:37), and equal gammas of the two sided
-    stencils, w(+x) / w(-x) = ((IS(q2) + eps) / (IS(q1) + eps))^4.  At unit
-    scale this pins the exponent and the indicator's coefficients; at 1e-6
-    (IS near eps) it pins eps."""
+def test_sided_weights_follow_the_indicator_ratio(orc, scale, axis):
+    """A block that is one quadratic q1 on the a >= 0 side of an axis a and
+    another, q2 = q1 + b a + c (a^2 - 1/12), on the a <= 0 side (the two have
+    the same cell averages on the a = 0 plane).  The +a stencil reads only
+    a >= 0 cells and the -a stencil only a <= 0 cells, so each reproduces its
+    quadratic and its indicator is that quadratic's derivative energy (sympy,
+    orders 1 and 2).  With the published weights gamma / (IS + eps)^4,
+    eps = 1e-12 (reconstruction.c:205-213, hype.h:37), and equal gammas of the
+    two sided stencils, w(+a) / w(-a) = ((IS(q2) + eps) / (IS(q1) + eps))^4.
+    At unit scale this pins the exponent and the indicator's coefficients; at
+    1e-6 (IS near eps) it pins eps."""
+    A, ax, wp = {"x": (X, 2, 3), "y": (Y, 1, 5), "z": (Z, 0, 7)}[axis]   # symbol, block axis, diag of w(+a)
     rng = np.random.default_rng(21)
     _, q1 = random_poly(rng, 2)
     b, c = rng.standard_normal(2)
     q1 = scale * q1
-    q2 = q1 + scale * (b * X + c * (X * X - R(1, 12)))
+    q2 = q1 + scale * (b * A + c * (A * A - R(1, 12)))
     blk = cell_averages_3d(q2)
-    blk[:, :, 2:] = cell_averages_3d(q1)[:, :, 2:]
+    side = [slice(None)] * 3
+    side[ax] = slice(2, None)
+    blk[tuple(side)] = cell_averages_3d(q1)[tuple(side)]
     _, dg = orc.cweno3d(blk, (0.85, 0.85))
     e1, e2 = derivative_energy(q1, 2), derivative_energy(q2, 2)
     want = ((e2 + 1e-12) / (e1 + 1e-12)) ** 4
-    got = dg[3] / dg[4]   # w(+x) / w(-x)
+    got = dg[wp] / dg[wp + 1]   # w(+a) / w(-a)
     assert abs(got / want - 1.0) < 1e-9, (got, want, e1, e2)
     # and the sides are told apart: the steeper side gets the smaller weight
     assert (got > 1.0) == (e2 > e1)
