@@ -76,16 +76,10 @@ static void fill_lin(LinArgs &a, const acx_grid *g)
 {
     memset(&a, 0, sizeof(a));
     a.nx = g->n[0]; a.ny = g->n[1]; a.nz = g->n[2];
-    // z-chunk per workgroup: 64 planes, halved (down to 8) while the launch
-    // would have fewer than 512 workgroups (two per CU), so small grids still
-    // fill the chip instead of running a few long serial marches
-    int zc = env_int("ACX_ZC_LINEAR", 0);
-    if (zc <= 0) {
-        const size_t cols = (size_t)((g->n[0] + 63) / 64) * ((g->n[1] + 7) / 8);
-        zc = 64;
-        while (zc > 8 && cols * ((g->n[2] + zc - 1) / zc) < 512) zc /= 2;
-    }
-    a.zc = zc < 1 ? 1 : (zc > g->n[2] ? g->n[2] : zc);
+    // z-chunk per workgroup: ACX_ZC_LINEAR, else 0 = chosen by the launcher
+    // for the kernel's tile and occupancy (launch_linear)
+    const int zc = env_int("ACX_ZC_LINEAR", 0);
+    a.zc = zc < 1 ? 0 : (zc > g->n[2] ? g->n[2] : zc);
     for (int d = 0; d < 3; ++d) a.bc[d] = g->bc[d];
     a.K = g->rho0 * g->c0 * g->c0;
     a.invrho = 1.0 / g->rho0;

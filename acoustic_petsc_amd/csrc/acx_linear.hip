@@ -40,13 +40,68 @@ k_linear(LinArgs p)
 
 // ---------------------------------------------------------------------------
 
+// Automatic z-chunk.  A launch runs in ceil(workgroups / slots) rounds
+// (slots = CUs x workgroups per CU), each about as long as one workgroup's
+// march of zc planes plus a fixed start (the 6 z-halo planes and the
+// prologue, ~10 planes' worth); pick the zc of 8 .. 128 that minimises
+// rounds x (zc + 10), the larger on a tie.  Measured (ACOUSTIC4, 256^3 and
+// 384^3): this model ranks the z-chunks as the timings do for the FP64
+// halo-ring kernel (one workgroup per CU: 128 planes at 256^3, +4 % over 64;
+// 64 at 384^3) and the FP32 lead-plane kernel (two per CU: 64 at 256^3,
+// 128 at 384^3, +3 % over 64).
+static int auto_zc(int ntx, int nty, int nz, int slots)
+{
+    static const int cand[] = {8, 16, 32, 64, 96, 128};
+    int best = nz < 8 ? nz : 8;
+    double best_cost = -1.0;
+    for (int zc : cand) {
+        if (zc > nz && zc != 8) break;
+        const int z = zc < nz ? zc : nz;
+        const long long wg = (long long)ntx * nty * ((nz + z - 1) / z);
+        const long long rounds = (wg + slots - 1) / slots;
+        const double cost = (double)rounds * (z + 10);
+        if (best_cost < 0 || cost <= best_cost) {
+            best_cost = cost;
+            best = z;
+        }
+    }
+    return best;
+}
+
+// CUs of the current device (cached per device).
+static int device_cus()
+{
+    static int cus[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cus[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev] = n;
+    }
+    return cus[dev];
+}
+
 template <typename T, int SYS, int EPI, int PF, int WPE, int BX, int BY, bool WALLS = false, bool NT = false>
 static int launch_linear_v(const LinArgs &a, hipStream_t s)
 {
     LinArgs b = a;
     b.ntx = (a.nx + BX - 1) / BX;
     b.nty = (a.ny + BY - 1) / BY;
-    b.ntz = (a.nz + a.zc - 1) / a.zc;
+    if (b.zc <= 0) {
+        // workgroups of this instantiation that fit a CU at once (queried once)
+        static int per_cu = 0;
+        if (!per_cu) {
+            int n = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &n, reinterpret_cast<const void *>(&k_linear<T, SYS, BX, BY, EPI, PF, WPE, WALLS, NT>), BX * BY, 0) !=
+                    hipSuccess || n <= 0)
+                n = 1;
+            per_cu = n;
+        }
+        b.zc = auto_zc(b.ntx, b.nty, a.nz, device_cus() * per_cu);
+    }
+    b.ntz = (a.nz + b.zc - 1) / b.zc;
     dim3 blk(BX, BY, 1);
     dim3 grd(b.ntx * b.nty * b.ntz, 1, 1);
     hipLaunchKernelGGL((k_linear<T, SYS, BX, BY, EPI, PF, WPE, WALLS, NT>), grd, blk, 0, s, b);
