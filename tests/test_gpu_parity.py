@@ -199,3 +199,35 @@ def test_f32c_wave2_nonlinear_is_refused(acx, torch_gpu):
     st = acx.CompensatedState(grid, torch.zeros(grid.shape(), dtype=torch.float64, device="cuda"))
     with pytest.raises(acx.AcxError, match="ACOUSTIC4"):
         st.step(1e-3)
+
+
+@pytest.mark.parametrize("gamma", [(1.0, 0.0), (0.85, 0.85)])
+def test_z_chunk_does_not_change_results(gamma, acx, torch_gpu, monkeypatch):
+    """The z-chunk of a launch (chosen per launch from the occupancy model, or
+    forced by ACX_ZC_LINEAR / ACX_ZC_CWENO) only changes which workgroup marches
+    which planes: FP64 steps and FP32 compensated steps are bitwise equal for
+    every chunk, including chunks that do not divide n_z."""
+    torch = torch_gpu
+    n = (40, 36, 44)
+    grid = acx.Grid(n=n, h=1.0 / 40, system=acx.ACOUSTIC4, gamma_hi=gamma[0], gamma_lo=gamma[1])
+    U0 = torch.from_numpy(acx.field_gaussians(grid, 77, npulse=3, amp=(1e-4, 1e-3), sigma=(0.05, 0.1))).cuda()
+    dt = acx.cfl_dt(grid, 0.4)
+    knob = "ACX_ZC_LINEAR" if gamma == (1.0, 0.0) else "ACX_ZC_CWENO"
+    results = []
+    for zc in (None, 5, 16, 44):
+        print(f"[case] z-chunk {knob}={zc} gamma={gamma}", flush=True)
+        if zc is None:
+            monkeypatch.delenv(knob, raising=False)
+        else:
+            monkeypatch.setenv(knob, str(zc))
+        U = U0.clone()
+        w = (torch.empty_like(U), torch.empty_like(U))
+        for _ in range(2):
+            acx.ssp_rks3_step(grid, U, dt, w)
+        st = acx.CompensatedState(grid, U0)
+        for _ in range(2):
+            st.step(dt)
+        results.append((U, st.hi.clone(), st.lo.clone()))
+    for U, hi, lo in results[1:]:
+        assert torch.equal(U, results[0][0])
+        assert torch.equal(hi, results[0][1]) and torch.equal(lo, results[0][2])
