@@ -27,7 +27,9 @@ constexpr int ENS_BX = 64;
 constexpr int ENS_MAXP = 256;
 
 struct EnsConfig { int by, zc; };
-static const EnsConfig kConfigs[6] = {{8, 16}, {8, 32}, {8, 64}, {16, 16}, {16, 32}, {16, 64}};
+constexpr int ENS_NCONFIG = 8;
+static const EnsConfig kConfigs[ENS_NCONFIG] = {{8, 16}, {8, 32}, {8, 64}, {16, 16}, {16, 32}, {16, 64},
+                                                {8, 128}, {16, 128}};
 
 // EPI: EPI_F32C for stages 1-3, EPI_F32CF for the stage that ends the step.
 template <int BY, int EPI>
@@ -171,7 +173,7 @@ static int ens_step(acx_ensemble *e, double dt, hipStream_t s)
 
 extern "C" {
 
-int acx_ensemble_num_configs(void) { return 6; }
+int acx_ensemble_num_configs(void) { return ENS_NCONFIG; }
 
 int acx_ensemble_create(int nprob, const int32_t *n, double h, acx_ensemble **out)
 {
@@ -275,7 +277,7 @@ int acx_ensemble_copy_out(const acx_ensemble *e, float *hi, float *lo, void *str
 
 int acx_ensemble_set_config(acx_ensemble *e, int config)
 {
-    if (!e || config < 0 || config >= 6) return set_error(ACX_EINVAL, "ensemble: config out of range");
+    if (!e || config < 0 || config >= ENS_NCONFIG) return set_error(ACX_EINVAL, "ensemble: config out of range");
     e->config = config;
     return ens_layout(e);
 }
@@ -340,7 +342,7 @@ int acx_ensemble_autotune(acx_ensemble *e, double dt, void *stream, int *best, f
     hipEventCreate(&b);
     float bestms = 1e30f;
     int bestc = e->config;
-    for (int c = 0; c < 6 && !rc; ++c) {
+    for (int c = 0; c < ENS_NCONFIG && !rc; ++c) {
         if ((rc = acx_ensemble_set_config(e, c))) break;
         if ((rc = ens_step(e, dt, s))) break;   // warm-up
         hipEventRecord(a, s);

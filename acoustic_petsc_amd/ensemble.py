@@ -23,6 +23,7 @@ def _bind():
         "acx_ensemble_step": [vp, dbl, vp],
         "acx_ensemble_graph_steps": [vp, dbl, i32, vp],
         "acx_ensemble_autotune": [vp, dbl, vp, _C.POINTER(i32), _C.POINTER(_C.c_float)],
+        "acx_ensemble_num_configs": [],
     }.items():
         fn = getattr(L, name)
         fn.argtypes = args
@@ -104,6 +105,11 @@ class Ensemble:
 
     def config(self):
         return _bind().acx_ensemble_get_config(self._h)
+
+    @staticmethod
+    def num_configs():
+        """Number of tile configurations (set_config / autotune)."""
+        return _bind().acx_ensemble_num_configs()
 
     def autotune(self, dt, stream=None):
         best, ms = _C.c_int(), _C.c_float()

@@ -39,7 +39,8 @@ int acx_ensemble_state(acx_ensemble *e, float **hi, float **lo);
 int acx_ensemble_copy_in(acx_ensemble *e, const float *hi, const float *lo, void *stream);
 int acx_ensemble_copy_out(const acx_ensemble *e, float *hi, float *lo, void *stream);
 
-/* tile configurations: 0..5 = (rows per tile 8 or 16) x (z-chunk 16, 32, 64) */
+/* tile configurations (acx_ensemble_num_configs() of them): 0..5 = (rows per
+ * tile 8 or 16) x (z-chunk 16, 32, 64), 6 = (8, 128), 7 = (16, 128) */
 int acx_ensemble_set_config(acx_ensemble *e, int config);
 int acx_ensemble_get_config(const acx_ensemble *e);
 int acx_ensemble_num_configs(void);

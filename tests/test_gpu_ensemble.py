@@ -38,7 +38,7 @@ def test_ensemble_equals_single_problem_stepper(acx, torch_gpu):
     dt = 0.4 * h / 3
     U0s = fields(acx, DIMS, h, 10)
     ref = single_steps(acx, torch, DIMS, h, U0s, dt, 3)
-    for cfg in range(6):
+    for cfg in range(Ensemble.num_configs()):
         print(f"[case] ensemble eager config {cfg}", flush=True)
         e = Ensemble(DIMS, h)
         e.set_config(cfg)
@@ -92,7 +92,7 @@ def test_autotune_restores_state(acx, torch_gpu):
     best, ms = e.autotune(dt)
     after = e.state32()
     print(f"[case] autotune picked {best} ({ms:.3f} ms/step)", flush=True)
-    assert 0 <= best < 6 and e.config() == best and ms > 0
+    assert 0 <= best < e.num_configs() and e.config() == best and ms > 0
     assert np.array_equal(before[0].cpu().numpy(), after[0].cpu().numpy())
     e.close()
 
