@@ -147,8 +147,9 @@ k_ctcs(const double *__restrict__ Pold, const double *__restrict__ P, double *__
     const int nx = c.n[0], ny = c.n[1], nz = c.n[2];
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
     if (i >= nx) return;
-    // periodic wrap; an axis of 1 cell (2D / 1D programs) maps every offset to 0
-    auto W = [](int a, int n) { a %= n; return a < 0 ? a + n : a; };
+    // periodic wrap of offsets within 2 cells (axes have 1 or at least 5
+    // cells); an axis of 1 cell (2D / 1D programs) maps every offset to 0
+    auto W = [](int a, int n) { return n == 1 ? 0 : (a < 0 ? a + n : (a >= n ? a - n : a)); };
     auto at = [&](int a, int b, int e) { return P[((size_t)W(e, nz) * ny + W(b, ny)) * nx + W(a, nx)]; };
     const double invdx2 = 1.0 / (c.dx * c.dx), invdy2 = 1.0 / (c.dy * c.dy), invdz2 = 1.0 / (c.dz * c.dz);
     const double p0 = at(i, j, k);

@@ -161,7 +161,7 @@ def cpu_baseline(acx, n_sample=96, max_steps=24, budget_s=10.0):
             "host": {"cpu": _host_cpu_model(), "available_cores": avail, "nproc": os.cpu_count()}}
 
 
-VARIANTS = ("fp64_cweno", "fp32c", "learned", "ensemble", "2d")
+VARIANTS = ("fp64_cweno", "fp32c", "learned", "ensemble", "2d", "ctcs")
 
 
 def pulse_2d(N, seed=1001, amp=1e-3, sigma=0.05):
@@ -200,6 +200,41 @@ def variant_2d(torch, acx, steps=200, N=256):
     t, _ = time_steps(torch, lambda: acx.ssp_rks3_step(g, U, dt, work), steps, 2)
     return {"value": N * N * STAGES * steps / t, "ms_per_step": 1e3 * t / steps, "steps": steps,
             "grid": [N, N, 1], "note": "launch-bound: 65536 cells per stage launch"}
+
+
+def variant_ctcs(torch, acx, steps=100):
+    """The CTCS leapfrog sibling (3d/ctcs/point_source/wave.c:138-215): 4th-order
+    13-point Laplacian + point source, FP64, on the reference's 100^3 problem
+    and at 256^3 (same parameters, source and receiver at the centre).  One
+    update reads Pold, P and the speed C and writes Pnew: 32 B per cell."""
+    out = {}
+    for n in (100, 256):
+        p = acx.Ctcs()
+        p.n, p.stencil = (n, n, n), 2   # stencil width 2: the 4th-order 13-point Laplacian
+        p.src = p.rcv = (n // 2, n // 2, n // 2)
+        cs = p.c_struct()
+        shape = (n, n, n)
+        Pold = torch.zeros(shape, dtype=torch.float64, device="cuda")
+        P, Pnew = torch.zeros_like(Pold), torch.zeros_like(Pold)
+        C = torch.full_like(Pold, p.c0)
+        bufs = [Pold, P, Pnew]
+        it = [0]
+
+        def step():
+            a, b, c = bufs
+            acx._check(acx.lib().acx_ctcs_update_f64(acx._C.byref(cs), acx._ptr(a, torch.float64),
+                                                     acx._ptr(b, torch.float64), acx._ptr(c, torch.float64),
+                                                     acx._ptr(C, torch.float64), it[0], None, acx._stream()))
+            bufs[:] = [b, c, a]
+            it[0] += 1
+
+        t, _ = time_steps(torch, step, steps, 5)
+        v = n ** 3 * steps / t
+        out[f"n{n}"] = {"value": v, "ms_per_step": 1e3 * t / steps, "steps": steps, "grid": [n, n, n],
+                        "bytes_per_cell_update": 32, "hbm_frac": v * 32 / 1e9 / HBM_PEAK_GBS}
+        del Pold, P, Pnew, C, bufs
+        torch.cuda.empty_cache()
+    return out
 
 
 def variant_fp64_cweno(torch, acx, N, U0, work, dt, steps):
@@ -445,6 +480,8 @@ def main():
             var["ensemble_48"] = variant_ensemble(torch, acx, args.steps)
         if "2d" in want:
             var["acoustic2d_256"] = variant_2d(torch, acx)
+        if "ctcs" in want:
+            var["ctcs_leapfrog"] = variant_ctcs(torch, acx)
         out["variants"] = var
 
     if not args.no_cpu:
