@@ -174,6 +174,96 @@ k_ctcs(const double *__restrict__ Pold, const double *__restrict__ P, double *__
     Pnew[o] = pn;
 }
 
+// The same update as a 2.5D z-march for 3D grids (every axis >= 5 cells): a
+// block owns a 32 x 8 column tile and a chunk of zc planes; each thread
+// keeps its column's planes z-ST .. z+ST in registers (the z part of the
+// Laplacian) and plane z sits in LDS with an ST-cell x/y halo (double
+// buffered, one barrier per plane), so P is read about once per update
+// instead of through 13 scattered loads.  Arithmetic as in k_ctcs, term by
+// term.  zc: planes per block (32, fewer for small grids, see the launcher).
+constexpr int CT_BX = 32, CT_BY = 8;
+
+template <int ST>
+__global__ void __launch_bounds__(CT_BX *CT_BY)
+k_ctcs_z(const double *__restrict__ Pold, const double *__restrict__ P, double *__restrict__ Pnew,
+         const double *__restrict__ C, acx_ctcs c, int it, int zc, double *__restrict__ rcv)
+{
+    constexpr int LX = CT_BX + 2 * ST, LY = CT_BY + 2 * ST, NH = 2 * ST * (CT_BX + CT_BY);
+    static_assert(NH <= CT_BX * CT_BY, "every halo cell of a plane needs a thread of its own");
+    __shared__ double pl[2][LY][LX];
+    const int nx = c.n[0], ny = c.n[1], nz = c.n[2];
+    const int ntx = (nx + CT_BX - 1) / CT_BX, nty = (ny + CT_BY - 1) / CT_BY;
+    const int b = blockIdx.x;
+    const int x0 = (b % ntx) * CT_BX, y0 = ((b / ntx) % nty) * CT_BY, zb = (b / (ntx * nty)) * zc;
+    const int ze = min(zb + zc, nz);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * CT_BX + tx;
+    const int i = x0 + tx, j = y0 + ty;
+    const bool active = i < nx && j < ny;
+    auto M = [](int a, int n) { a %= n; return a < 0 ? a + n : a; };   // set-up only
+    auto Wz = [nz](int a) { return a < 0 ? a + nz : (a >= nz ? a - nz : a); };   // |offset| <= ST + 1 < nz
+    const size_t plane = (size_t)nx * ny;
+    // columns past the grid edge hold the periodic image, so the last tile's
+    // LDS row also serves as the wrapped x/y neighbours of the edge cells
+    const size_t col = (size_t)M(j, ny) * nx + M(i, nx);
+    int hx = -1, hy = -1;
+    size_t hcol = 0;
+    if (tid < NH) {
+        int t = tid;
+        if (t < 2 * ST * CT_BY) {
+            const int row = t / (2 * ST), k = t % (2 * ST);
+            hy = row + ST;
+            hx = k < ST ? k : CT_BX + k;
+        } else {
+            t -= 2 * ST * CT_BY;
+            const int cx = t % CT_BX, k = t / CT_BX;
+            hx = cx + ST;
+            hy = k < ST ? k : CT_BY + k;
+        }
+        hcol = (size_t)M(y0 + hy - ST, ny) * nx + M(x0 + hx - ST, nx);
+    }
+    const double invdx2 = 1.0 / (c.dx * c.dx), invdy2 = 1.0 / (c.dy * c.dy), invdz2 = 1.0 / (c.dz * c.dz);
+    const double dt = c.dt;
+    double q[2 * ST + 1];
+#pragma unroll
+    for (int d = -ST; d <= ST; ++d) q[d + ST] = P[(size_t)Wz(zb + d) * plane + col];
+    int buf = 0;
+    for (int z = zb; z < ze; ++z) {
+        const double nxt = P[(size_t)Wz(z + ST + 1) * plane + col];   // enters the queue at the end
+        const size_t o = (size_t)z * plane + col;
+        const double po = Pold[o], cc = C[o];
+        pl[buf][ty + ST][tx + ST] = q[ST];
+        if (hx >= 0) pl[buf][hy][hx] = P[(size_t)z * plane + hcol];
+        __syncthreads();
+        if (active) {
+            const double p0 = q[ST];
+            const double *row = &pl[buf][ty + ST][tx + ST];
+            double pxx, pyy, pzz;
+            if (ST == 1) {
+                pxx = (row[1] - 2.0 * p0 + row[-1]) * invdx2;
+                pyy = (pl[buf][ty + ST + 1][tx + ST] - 2.0 * p0 + pl[buf][ty + ST - 1][tx + ST]) * invdy2;
+                pzz = (q[2] - 2.0 * p0 + q[0]) * invdz2;
+            } else {
+                pxx = (-1.0 / 12 * row[2] + 4.0 / 3.0 * row[1] - 5.0 / 2.0 * p0 + 4.0 / 3.0 * row[-1] -
+                       1.0 / 12.0 * row[-2]) * invdx2;
+                pyy = (-1.0 / 12 * pl[buf][ty + ST + 2][tx + ST] + 4.0 / 3.0 * pl[buf][ty + ST + 1][tx + ST] -
+                       5.0 / 2.0 * p0 + 4.0 / 3.0 * pl[buf][ty + ST - 1][tx + ST] -
+                       1.0 / 12.0 * pl[buf][ty + ST - 2][tx + ST]) * invdy2;
+                pzz = (-1.0 / 12 * q[2 * ST] + 4.0 / 3.0 * q[ST + 1] - 5.0 / 2.0 * p0 + 4.0 / 3.0 * q[ST - 1] -
+                       1.0 / 12.0 * q[0]) * invdz2;
+            }
+            double pn = 2.0 * p0 - po + cc * cc * dt * dt * (pxx + pyy + pzz);
+            if (i == c.src[0] && j == c.src[1] && z == c.src[2])
+                pn += ctcs_source(it * dt, c.f0) / (c.dx * c.dy * c.dz) * dt * dt;
+            if (i == c.rcv[0] && j == c.rcv[1] && z == c.rcv[2]) rcv[0] = p0;
+            Pnew[o] = pn;
+        }
+#pragma unroll
+        for (int d = 0; d < 2 * ST; ++d) q[d] = q[d + 1];
+        q[2 * ST] = nxt;
+        buf ^= 1;
+    }
+}
+
 // ---- host fields ---------------------------------------------------------
 
 static inline uint64_t splitmix64(uint64_t &s)
@@ -264,9 +354,21 @@ int acx_ctcs_update_f64(const acx_ctcs *c, const double *POLD, const double *P, 
     if (!part) return set_error(ACX_EHIP, "ctcs: scratch allocation failed");
     hipStream_t s = (hipStream_t)stream;
     double *rcv = part + 3 * RED_BLOCKS + 3;
-    dim3 blk(256), grd((c->n[0] + 255) / 256, c->n[1], c->n[2]);
-    if (c->stencil == 1) hipLaunchKernelGGL(k_ctcs<1>, grd, blk, 0, s, POLD, P, PNEW, C, *c, it, rcv);
-    else hipLaunchKernelGGL(k_ctcs<2>, grd, blk, 0, s, POLD, P, PNEW, C, *c, it, rcv);
+    if (c->n[0] >= 5 && c->n[1] >= 5 && c->n[2] >= 5) {
+        // 3D: z-march; 32 planes per block, halved (down to 4) below 1024 blocks
+        const size_t cols = (size_t)((c->n[0] + CT_BX - 1) / CT_BX) * ((c->n[1] + CT_BY - 1) / CT_BY);
+        int zc = 32;
+        while (zc > 4 && cols * ((c->n[2] + zc - 1) / zc) < 1024) zc /= 2;
+        const size_t nb = cols * ((c->n[2] + zc - 1) / zc);
+        dim3 blk(CT_BX, CT_BY), grd((unsigned)nb);
+        if (c->stencil == 1) hipLaunchKernelGGL(k_ctcs_z<1>, grd, blk, 0, s, POLD, P, PNEW, C, *c, it, zc, rcv);
+        else hipLaunchKernelGGL(k_ctcs_z<2>, grd, blk, 0, s, POLD, P, PNEW, C, *c, it, zc, rcv);
+    } else {
+        // 2D / 1D programs (axes of one cell): one thread per cell
+        dim3 blk(256), grd((c->n[0] + 255) / 256, c->n[1], c->n[2]);
+        if (c->stencil == 1) hipLaunchKernelGGL(k_ctcs<1>, grd, blk, 0, s, POLD, P, PNEW, C, *c, it, rcv);
+        else hipLaunchKernelGGL(k_ctcs<2>, grd, blk, 0, s, POLD, P, PNEW, C, *c, it, rcv);
+    }
     int rc = hip_check(hipGetLastError(), "ctcs launch");
     if (rc) return rc;
     if (rcv_out) {
