@@ -161,7 +161,7 @@ def cpu_baseline(acx, n_sample=96, max_steps=24, budget_s=10.0):
             "host": {"cpu": _host_cpu_model(), "available_cores": avail, "nproc": os.cpu_count()}}
 
 
-VARIANTS = ("fp64_cweno", "fp32c", "learned", "ensemble", "2d", "ctcs")
+VARIANTS = ("fp64_cweno", "fp32c", "learned", "ensemble", "2d", "ctcs", "wave2")
 
 
 def pulse_2d(N, seed=1001, amp=1e-3, sigma=0.05):
@@ -200,6 +200,27 @@ def variant_2d(torch, acx, steps=200, N=256):
     t, _ = time_steps(torch, lambda: acx.ssp_rks3_step(g, U, dt, work), steps, 2)
     return {"value": N * N * STAGES * steps / t, "ms_per_step": 1e3 * t / steps, "steps": steps,
             "grid": [N, N, 1], "note": "launch-bound: 65536 cells per stage launch"}
+
+
+def variant_wave2(torch, acx, N=256, steps=20):
+    """The literal 3d/weno4/plane_wave system (u, v; du/dt = v, dv/dt = c^2
+    div grad u) at N^3, FP64, SSP-RK3 at CFL 0.4, 4 seeded Gaussian pulses in
+    u: the shipped linear weights (HBM-bound, 9/4 x 2 x 8 = 36 B per
+    cell-update) and the nonlinear 0.85/0.85 weights."""
+    out = {}
+    for name, gam in (("linear", (1.0, 0.0)), ("nonlinear_g085", (0.85, 0.85))):
+        g = acx.Grid(n=(N, N, N), h=1.0 / N, system=acx.WAVE2, gamma_hi=gam[0], gamma_lo=gam[1])
+        U = torch.from_numpy(acx.field_gaussians(g, 2025)).cuda()
+        work = (torch.empty_like(U), torch.empty_like(U))
+        dt = acx.cfl_dt(g, 0.4)
+        k = steps if name == "linear" else max(2, steps // 4)
+        t, _ = time_steps(torch, lambda: acx.ssp_rks3_step(g, U, dt, work), k, 1)
+        v = g.ncells * STAGES * k / t
+        out[name] = {"value": v, "ms_per_step": 1e3 * t / k, "steps": k, "grid": [N, N, N],
+                     "bytes_per_cell_update": 36, "hbm_frac": v * 36 / 1e9 / HBM_PEAK_GBS}
+        del U, work
+        torch.cuda.empty_cache()
+    return out
 
 
 def variant_ctcs(torch, acx, steps=100):
@@ -482,6 +503,8 @@ def main():
             var["acoustic2d_256"] = variant_2d(torch, acx)
         if "ctcs" in want:
             var["ctcs_leapfrog"] = variant_ctcs(torch, acx)
+        if "wave2" in want:
+            var["wave2_256"] = variant_wave2(torch, acx)
         out["variants"] = var
 
     if not args.no_cpu:
