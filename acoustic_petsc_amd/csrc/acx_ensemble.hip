@@ -65,7 +65,8 @@ k_ensemble(const EnsProblem *__restrict__ probs, int nprob, int total, unsigned 
         a.out2 = base.out2 ? static_cast<float *>(base.out2) + o : nullptr;
         const int x0 = (local % pr.ntx) * ENS_BX, y0 = ((local / pr.ntx) % pr.nty) * BY;
         const int zb = (local / (pr.ntx * pr.nty)) * a.zc;
-        linear_tile<float, ACX_ACOUSTIC4, ENS_BX, BY, EPI, 1, false>(a, x0, y0, zb, lds);   // loads one plane ahead, periodic
+        // loads one plane ahead, periodic, results streamed past L2
+        linear_tile<float, ACX_ACOUSTIC4, ENS_BX, BY, EPI, 1, false, true>(a, x0, y0, zb, lds);
         __syncthreads();   // LDS is reused by the next item
     }
 }

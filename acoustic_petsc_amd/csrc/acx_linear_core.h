@@ -25,7 +25,8 @@ enum { EPI_RHS = 0, EPI_AXPY = 1, EPI_COMB = 2, EPI_F32C = 3, EPI_F32CF = 4 };
 // One tile: columns [x0, x0+BX) x [y0, y0+BY), planes [zb, ze) of the grid
 // described by p (shared by the per-grid kernel and the ensemble kernel).
 // WALLS = false: every axis periodic (no ghost signs, wrap-only index maps).
-template <typename T, int SYS, int BX, int BY, int EPI, int PF = 2, bool WALLS = true>
+// NT: results stored nontemporally (streamed past L2).
+template <typename T, int SYS, int BX, int BY, int EPI, int PF = 2, bool WALLS = true, bool NT = false>
 __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, int zb,
                                             T (&lds)[2][LinTile<T, SYS, BX, BY>::NS][BY + 6][BX + 6])
 {
@@ -295,11 +296,13 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
                             nh.v[v] = sm;
                             nl.v[v] = er;
                         }
-                        store_cell<T, NV>(OUT2, cidx, nl);
+                        if constexpr (NT) store_cell_nt<T, NV>(OUT2, cidx, nl);
+                        else store_cell<T, NV>(OUT2, cidx, nl);
                         o = nh;
                     }
                 }
-                store_cell<T, NV>(OUT, cidx, o);
+                if constexpr (NT) store_cell_nt<T, NV>(OUT, cidx, o);
+                else store_cell<T, NV>(OUT, cidx, o);
             }
             issue(S, j + PF);   // refill this slot for plane j + PF
         }
