@@ -119,7 +119,8 @@ static int launch_linear_v(const LinArgs &a, hipStream_t s)
 // with streaming epilogue (15); lead-plane 64 x 8 with loads 1 plane ahead
 // and nontemporal stores (20); any other value: lead-plane 64 x 8 with loads
 // 2 planes ahead.
-// Defaults (measured at 256^3 and 384^3): FP64 variant 14, FP32 variant 20
+// Defaults (measured at 256^3 and 384^3): FP64 variant 15 (the halo ring
+// with nontemporal epilogue traffic: +0.7-1.2 % over 14), FP32 variant 20
 // (variant 0 with nontemporal stores: +2.5 %).
 // Grids with a wall on any axis take the default variant of their precision
 // instantiated with ghost signs.
@@ -127,10 +128,10 @@ template <typename T, int SYS, int EPI>
 static int launch_linear_t(const LinArgs &a, hipStream_t s)
 {
     if (a.bc[0] != ACX_PERIODIC || a.bc[1] != ACX_PERIODIC || a.bc[2] != ACX_PERIODIC) {
-        if constexpr (sizeof(T) == 8) return launch_linear_v<T, SYS, EPI, -2, 1, 32, 16, true>(a, s);
+        if constexpr (sizeof(T) == 8) return launch_linear_v<T, SYS, EPI, -2, 1, 32, 16, true, true>(a, s);
         else return launch_linear_v<T, SYS, EPI, 1, 1, 64, 8, true, true>(a, s);
     }
-    static const int var = env_int("ACX_LIN_VARIANT", sizeof(T) == 8 ? 14 : 20);
+    static const int var = env_int("ACX_LIN_VARIANT", sizeof(T) == 8 ? 15 : 20);
     switch (var) {
     case 0: return launch_linear_v<T, SYS, EPI, 1, 1, 64, 8>(a, s);
     case 1: return launch_linear_v<T, SYS, EPI, 1, 4, 64, 8>(a, s);
