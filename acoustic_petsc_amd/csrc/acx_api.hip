@@ -28,6 +28,44 @@ int env_int(const char *name, int dflt)
     return (s && *s) ? atoi(s) : dflt;
 }
 
+// Automatic z-chunk of a 2.5D march.  A launch runs in ceil(workgroups /
+// slots) rounds (slots = CUs x workgroups that fit a CU), each about as long
+// as one workgroup's march of zc planes plus a fixed start of `start` planes'
+// worth (z-halo planes, prologue); the zc of 8 .. 128 that minimises
+// rounds x (zc + start) is taken, the larger on a tie.
+int auto_zc(int ntx, int nty, int nz, int slots, int start)
+{
+    static const int cand[] = {8, 16, 32, 48, 64, 96, 128};
+    int best = nz < 8 ? nz : 8;
+    double best_cost = -1.0;
+    for (int zc : cand) {
+        if (zc > nz && zc != 8) break;
+        const int z = zc < nz ? zc : nz;
+        const long long wg = (long long)ntx * nty * ((nz + z - 1) / z);
+        const long long rounds = (wg + slots - 1) / slots;
+        const double cost = (double)rounds * (z + start);
+        if (best_cost < 0 || cost <= best_cost) {
+            best_cost = cost;
+            best = z;
+        }
+    }
+    return best;
+}
+
+// CUs of the current device (cached per device).
+int device_cus()
+{
+    static int cus[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cus[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev] = n;
+    }
+    return cus[dev];
+}
+
 int hip_check(hipError_t e, const char *what)
 {
     if (e == hipSuccess) return ACX_OK;
@@ -92,17 +130,9 @@ static void fill_cw(CwArgs &a, const acx_grid *g)
 {
     memset(&a, 0, sizeof(a));
     a.nx = g->n[0]; a.ny = g->n[1]; a.nz = g->n[2];
-    // z-chunk: 64 planes, halved (down to 8) below 512 workgroups (two per
-    // CU; one fits at a time).  Measured at 256^3 and 384^3 (FP64, gamma =
-    // 0.85): 64 planes 3.7 % faster than 32 (fewer recomputed ring planes),
-    // 16 and 8 slower still.
-    int zc = env_int("ACX_ZC_CWENO", 0);
-    if (zc <= 0) {
-        const size_t cols = (size_t)((g->n[0] + 29) / 30) * ((g->n[1] + 13) / 14);
-        zc = 64;
-        while (zc > 8 && cols * ((g->n[2] + zc - 1) / zc) < 512) zc /= 2;
-    }
-    a.zc = zc < 1 ? 1 : (zc > g->n[2] ? g->n[2] : zc);
+    // z-chunk: ACX_ZC_CWENO, else 0 = chosen by the launcher (launch_cweno)
+    const int zc = env_int("ACX_ZC_CWENO", 0);
+    a.zc = zc < 1 ? 0 : (zc > g->n[2] ? g->n[2] : zc);
     for (int d = 0; d < 3; ++d) a.bc[d] = g->bc[d];
     a.K = g->rho0 * g->c0 * g->c0;
     a.invrho = 1.0 / g->rho0;

@@ -51,6 +51,9 @@ size_t cweno_sel_tiles(const CwArgs &a);
 int launch_cweno_tilemask(const CwArgs &a, const uint8_t *flags, uint8_t *mask, hipStream_t s);
 
 int set_error(int code, const char *fmt, ...);
+// z-chunk of a 2.5D march from the round model (acx_api.hip); CUs of the current device
+int auto_zc(int ntx, int nty, int nz, int slots, int start);
+int device_cus();
 // integer tuning knob from the environment (dflt when unset or empty)
 int env_int(const char *name, int dflt);
 int hip_check(hipError_t e, const char *what);

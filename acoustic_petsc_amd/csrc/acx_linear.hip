@@ -40,48 +40,11 @@ k_linear(LinArgs p)
 
 // ---------------------------------------------------------------------------
 
-// Automatic z-chunk.  A launch runs in ceil(workgroups / slots) rounds
-// (slots = CUs x workgroups per CU), each about as long as one workgroup's
-// march of zc planes plus a fixed start (the 6 z-halo planes and the
-// prologue, ~10 planes' worth); pick the zc of 8 .. 128 that minimises
-// rounds x (zc + 10), the larger on a tie.  Measured (ACOUSTIC4, 256^3 and
-// 384^3): this model ranks the z-chunks as the timings do for the FP64
-// halo-ring kernel (one workgroup per CU: 128 planes at 256^3, +4 % over 64;
-// 64 at 384^3) and the FP32 lead-plane kernel (two per CU: 64 at 256^3,
-// 128 at 384^3, +3 % over 64).
-static int auto_zc(int ntx, int nty, int nz, int slots)
-{
-    static const int cand[] = {8, 16, 32, 64, 96, 128};
-    int best = nz < 8 ? nz : 8;
-    double best_cost = -1.0;
-    for (int zc : cand) {
-        if (zc > nz && zc != 8) break;
-        const int z = zc < nz ? zc : nz;
-        const long long wg = (long long)ntx * nty * ((nz + z - 1) / z);
-        const long long rounds = (wg + slots - 1) / slots;
-        const double cost = (double)rounds * (z + 10);
-        if (best_cost < 0 || cost <= best_cost) {
-            best_cost = cost;
-            best = z;
-        }
-    }
-    return best;
-}
-
-// CUs of the current device (cached per device).
-static int device_cus()
-{
-    static int cus[64] = {};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    if (!cus[dev]) {
-        int n = 0;
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-        cus[dev] = n;
-    }
-    return cus[dev];
-}
-
+// Automatic z-chunk (auto_zc, acx_api.hip): measured for these kernels (ACOUSTIC4,
+// 256^3 and 384^3), the round model ranks the z-chunks as the timings do for
+// the FP64 halo-ring kernel (one workgroup per CU: 128 planes at 256^3, +4 %
+// over 64; 64 at 384^3) and the FP32 lead-plane kernel (two per CU: 64 at
+// 256^3, 128 at 384^3, +3 % over 64).
 template <typename T, int SYS, int EPI, int PF, int WPE, int BX, int BY, bool WALLS = false, bool NT = false>
 static int launch_linear_v(const LinArgs &a, hipStream_t s)
 {
@@ -99,7 +62,7 @@ static int launch_linear_v(const LinArgs &a, hipStream_t s)
                 n = 1;
             per_cu = n;
         }
-        b.zc = auto_zc(b.ntx, b.nty, a.nz, device_cus() * per_cu);
+        b.zc = auto_zc(b.ntx, b.nty, a.nz, device_cus() * per_cu, 10);
     }
     b.ntz = (a.nz + b.zc - 1) / b.zc;
     dim3 blk(BX, BY, 1);
