@@ -120,7 +120,9 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
             if constexpr (EPI == EPI_COMB) S.e0 = load_cell<T, NV>(U0, cidx);
             if constexpr (F32C) {
                 if (fin_stage) {
-                    S.e1 = load_cell<T, NV>(LO, cidx);
+                    // LO is read once: streamed past L2 with the nontemporal stores
+                    if constexpr (NT) S.e1 = load_cell_nt<T, NV>(LO, cidx);
+                    else S.e1 = load_cell<T, NV>(LO, cidx);
                     // HI of plane c, exact (EPI_F32CF: from the register queue)
                     if constexpr (EPI == EPI_F32C) S.e2 = load_cell<T, NV>(Q, cidx);
                 }
