@@ -77,20 +77,45 @@ def traffic_from_profiles(n):
     return None
 
 
-def time_steps(torch, step, steps, warmup):
+def time_steps(torch, step, steps, warmup, per_step=None):
+    """GPU time of `steps` calls of step() after `warmup` untimed ones, from
+    HIP events on the launch stream (torch's current stream, on which the
+    library launches).  With a dict `per_step`, an event is also recorded
+    between consecutive steps and the median / min / max step time (s) are
+    stored there."""
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1 if per_step is not None else 2)]
     t0 = time.perf_counter()
-    e0.record(s)
-    for _ in range(steps):
+    ev[0].record(s)
+    for i in range(steps):
         step()
-    e1.record(s)
+        if per_step is not None:
+            ev[i + 1].record(s)
+    ev[-1].record(s)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return e0.elapsed_time(e1) / 1e3, wall
+    if per_step is not None and steps > 0:
+        dts = sorted(ev[i].elapsed_time(ev[i + 1]) / 1e3 for i in range(steps))
+        per_step.update({"median_s": dts[len(dts) // 2], "min_s": dts[0], "max_s": dts[-1]})
+    return ev[0].elapsed_time(ev[-1]) / 1e3, wall
+
+
+def settle(torch, step, batch=10, tol=0.01, max_s=3.0):
+    """Runs untimed batches of `batch` steps until the GPU time of two
+    consecutive batches agrees within `tol` (clocks and caches at steady
+    state), at most `max_s` seconds.  Returns (steps run, last batch times)."""
+    n, last, t_end = 0, None, time.perf_counter() + max_s
+    while True:
+        t, _ = time_steps(torch, step, batch, 0)
+        n += batch
+        if last is not None and abs(t - last) <= tol * min(t, last):
+            return n, (last, t)
+        if time.perf_counter() > t_end:
+            return n, (last, t)
+        last = t
 
 
 def _host_cpu_model():
@@ -303,7 +328,8 @@ def variant_fp32c(torch, acx, M, steps, fp64_value, err_steps=200):
 def train_indicator(torch, acx):
     """The troubled-cell MLP trained on the GPU by the library's own kernels:
     8 solver-labelled snapshots of 64^3 steep-pulse runs (2.1e6 cells, 8.4e6
-    (cell, component) samples), label |wt_ratio - 1| > 0.05."""
+    (cell, component) samples), labelled positive where the eight normalised
+    CWENO4 weights leave the linear weights, sum_k |w_k - gamma_k| > 0.05."""
     from acoustic_petsc_amd import learn
     t0 = time.perf_counter()
     X, T = learn.make_dataset(n_snapshots=8, N=64, seed0=100)
@@ -453,7 +479,10 @@ def main():
         torch.cuda.synchronize()
         del tmp
 
-    t_gpu, wall = time_steps(torch, lambda: acx.ssp_rks3_step(grid, U, dt, work), args.steps, args.warmup)
+    headline_step = lambda: acx.ssp_rks3_step(grid, U, dt, work)
+    settle_steps, settle_pair = settle(torch, headline_step)
+    per_step = {}
+    t_gpu, wall = time_steps(torch, headline_step, args.steps, args.warmup, per_step)
     cells = grid.ncells
     value = cells * STAGES * args.steps / t_gpu
     launch_s = t_gpu / (STAGES * args.steps)
@@ -476,11 +505,22 @@ def main():
                                f"FP64, SSP-RK3 4 stages, CFL 0.4, 4 seeded Gaussian pulses",
                    "grid": [N, N, N], "gamma": [1.0, 0.0], "precision": "fp64", "steps": args.steps,
                    "wall_s": wall},
+        "timing": {"settle_steps": settle_steps,
+                   "settle_last_batches_ms": [1e3 * x if x is not None else None for x in settle_pair],
+                   "step_ms_median": 1e3 * per_step.get("median_s", float("nan")),
+                   "step_ms_min": 1e3 * per_step.get("min_s", float("nan")),
+                   "step_ms_max": 1e3 * per_step.get("max_s", float("nan")),
+                   "note": "untimed batches of 10 steps run until two consecutive batches agree "
+                           "within 1 %, then the warmup steps, then the timed steps"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
+                     "traffic_source": f"profiles/k_linear_fp64_ac4_{N}_pmc.md (FETCH_SIZE/WRITE_SIZE "
+                                       "of a rocprofv3 --pmc run, not measured in this run)"
+                     if traffic is not None else None,
                      "kernel": "k_linear<double, ACOUSTIC4> (fused RHS + stage)",
-                     "bytes_per_cell_update": BYTES_FP64_AC4},
+                     "bytes_per_cell_update": BYTES_FP64_AC4,
+                     "bytes_per_launch": BYTES_FP64_AC4 * cells},
     }
 
     if args.variants != "none":

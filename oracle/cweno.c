@@ -40,6 +40,12 @@ static double smooth_q(const double d[NQ])
 void orc_cweno3d(const double *c, ptrdiff_t sx, ptrdiff_t sy, ptrdiff_t sz,
                  double gamma_hi, double gamma_lo, double co[20], double *diag)
 {
+    orc_cweno3d_ext(c, sx, sy, sz, gamma_hi, gamma_lo, co, diag, NULL);
+}
+
+void orc_cweno3d_ext(const double *c, ptrdiff_t sx, ptrdiff_t sy, ptrdiff_t sz,
+                     double gamma_hi, double gamma_lo, double co[20], double *diag, double *dstencil)
+{
 #define V(a, b, e) c[(a) * sx + (b) * sy + (e) * sz]
     const double u0 = V(0, 0, 0);
     /* axis arms */
@@ -184,6 +190,9 @@ void orc_cweno3d(const double *c, ptrdiff_t sx, ptrdiff_t sy, ptrdiff_t sz,
         for (int s = 0; s < 7; ++s) diag[2 + s] = w3[s];
         diag[9] = is4;
     }
+    if (dstencil)   /* the 9 derivative estimates of each quadratic stencil */
+        for (int s = 0; s < 7; ++s)
+            for (int q = 0; q < NQ; ++q) dstencil[s * NQ + q] = d[s][q];
     (void)k_1_3;
 }
 
@@ -242,6 +251,12 @@ static double pow4(double a) { double a2 = a * a; return a2 * a2; }
 
 void orc_cweno2d(const double *c, ptrdiff_t sx, ptrdiff_t sy, double gamma_hi, double gamma_lo,
                  double co[10])
+{
+    orc_cweno2d_ext(c, sx, sy, gamma_hi, gamma_lo, co, NULL, NULL);
+}
+
+void orc_cweno2d_ext(const double *c, ptrdiff_t sx, ptrdiff_t sy, double gamma_hi, double gamma_lo,
+                     double co[10], double *diag, double *dstencil)
 {
 #define V(a, b) c[(a) * sx + (b) * sy]
     const double u0 = V(0, 0);
@@ -330,4 +345,19 @@ void orc_cweno2d(const double *c, ptrdiff_t sx, ptrdiff_t sy, double gamma_hi, d
     co[7] = wt * uyyy;
     co[8] = wt * uxxy;
     co[9] = wt * uxyy;
+
+    if (diag) {
+        diag[0] = wt;
+        diag[1] = w4;
+        for (int s = 0; s < 5; ++s) diag[2 + s] = w3[s];
+        diag[7] = is4;
+    }
+    if (dstencil)   /* x, y, xx, yy, xy of each quadratic stencil */
+        for (int s = 0; s < 5; ++s) {
+            dstencil[5 * s + 0] = dx[s];
+            dstencil[5 * s + 1] = dy[s];
+            dstencil[5 * s + 2] = dxx[s];
+            dstencil[5 * s + 3] = dyy[s];
+            dstencil[5 * s + 4] = dxy[s];
+        }
 }

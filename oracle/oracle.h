@@ -53,6 +53,11 @@ int orc_nvar(int system);
  * [0] wt_ratio, [1] w_R4, [2..8] w_R3[0..6], [9] IS_R4. */
 void orc_cweno3d(const double *c, ptrdiff_t sx, ptrdiff_t sy, ptrdiff_t sz,
                  double gamma_hi, double gamma_lo, double coeffs[20], double *diag);
+/* The same, also writing (dstencil, may be NULL) the 9 derivative estimates
+ * x, y, z, xx, yy, zz, xy, yz, xz of each of the 7 quadratic stencils in the
+ * order central, +x, -x, +y, -y, +z, -z (reconstruction.c:121-203): 63 numbers. */
+void orc_cweno3d_ext(const double *c, ptrdiff_t sx, ptrdiff_t sy, ptrdiff_t sz,
+                     double gamma_hi, double gamma_lo, double coeffs[20], double *diag, double *dstencil);
 
 /* reconstruction.c:12-21 and :24-34 */
 double orc_poly3d(double x, double y, double z, const double coeffs[20]);
@@ -126,6 +131,11 @@ void orc_weno2d_linear(const double *c, ptrdiff_t sx, ptrdiff_t sy, double coeff
 /* 2d/weno4/bubble/reconstruction.c:128-251 (CWENO4, gamma_hi/gamma_lo) */
 void orc_cweno2d(const double *c, ptrdiff_t sx, ptrdiff_t sy, double gamma_hi, double gamma_lo,
                  double coeffs[10]);
+/* The same with diag (may be NULL: [0] wt_ratio, [1] w_R4, [2..6] w_R3 of the
+ * stencils central, -x, +x, -y, +y, [7] IS_R4) and dstencil (may be NULL:
+ * x, y, xx, yy, xy of each of the 5 stencils in that order, 25 numbers). */
+void orc_cweno2d_ext(const double *c, ptrdiff_t sx, ptrdiff_t sy, double gamma_hi, double gamma_lo,
+                     double coeffs[10], double *diag, double *dstencil);
 
 typedef struct {
     int    N[2];
