@@ -115,6 +115,7 @@ def lib():
             "acx_rhs_f64": [gp, vp, vp, vp],
             "acx_stage_f64": [gp, vp, vp, vp, dbl, dbl, dbl, vp],
             "acx_ssp_rks3_step_f64": [gp, vp, vp, vp, dbl, vp],
+            "acx_ssp_rks3_steps_graph_f64": [gp, vp, vp, vp, dbl, i32, vp],
             "acx_ssp_rks3_step_f32c": [gp, vp, vp, vp, vp, vp, vp, dbl, vp],
             "acx_ssp_rks3_step_f32c_sel": [gp, vp, vp, vp, vp, vp, vp, vp, dbl, vp],
             "acx_ssp_rks3_step_f32c_selm": [gp, vp, vp, vp, vp, vp, vp, vp, vp, dbl, vp],
@@ -136,6 +137,10 @@ def lib():
         L.acx_cfl_dt.restype = dbl
         L.acx_sel_mask_bytes.argtypes = [gp]
         L.acx_sel_mask_bytes.restype = _C.c_size_t
+        L.acx_nonfinite_check.argtypes = [vp]
+        L.acx_nonfinite_check.restype = i32
+        L.acx_nonfinite_clear.argtypes = []
+        L.acx_nonfinite_clear.restype = i32
         _lib = L
     return _lib
 
@@ -196,6 +201,16 @@ def ssp_rks3_step(grid, U, dt, work=None, stream=None):
     g = grid.c_struct()
     _check(lib().acx_ssp_rks3_step_f64(_C.byref(g), _ptr(U, torch.float64), _ptr(work[0], torch.float64),
                                        _ptr(work[1], torch.float64), dt, _stream(stream)))
+    return U
+
+
+def ssp_rks3_steps_graph(grid, U, dt, nsteps, work, stream=None):
+    """nsteps TSSSPRKS3 steps in place, replayed from a captured hipGraph of
+    the four stage launches (captured once per grid, arrays and dt)."""
+    torch = _torch()
+    g = grid.c_struct()
+    _check(lib().acx_ssp_rks3_steps_graph_f64(_C.byref(g), _ptr(U, torch.float64), _ptr(work[0], torch.float64),
+                                              _ptr(work[1], torch.float64), dt, int(nsteps), _stream(stream)))
     return U
 
 
@@ -289,6 +304,19 @@ def energy(grid, U, stream=None):
     g = grid.c_struct()
     _check(lib().acx_energy_f64(_C.byref(g), _ptr(U, torch.float64), out, _stream(stream)))
     return out[0]
+
+
+def nonfinite_check(stream=None):
+    """Synchronises the stream and raises AcxError (code 4, ACX_ENONFINITE)
+    when a stage kernel on this device has written a NaN or Inf since the
+    last nonfinite_clear().  RHS / stage / step calls raise the same error at
+    entry once the flag is visible, without synchronising."""
+    _check(lib().acx_nonfinite_check(_stream(stream)))
+
+
+def nonfinite_clear():
+    """Resets the non-finite watch of the current device."""
+    _check(lib().acx_nonfinite_clear())
 
 
 def cfl_dt(grid, cfl):

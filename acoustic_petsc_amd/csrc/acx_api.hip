@@ -72,6 +72,63 @@ int hip_check(hipError_t e, const char *what)
     return set_error(ACX_EHIP, "%s: %s", what, hipGetErrorString(e));
 }
 
+// Non-finite watch: one pinned, device-mapped host word per device.  Stage
+// kernels store 1 into it when they write a NaN or +-Inf (watch_finite in
+// acx_device.h); the host reads it without synchronising.
+struct NfWord {
+    volatile unsigned *host;
+    unsigned *dev;
+};
+static NfWord g_nf[64];
+static std::mutex g_nf_mu;
+
+static NfWord *nf_for_current(bool create)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    NfWord &w = g_nf[dev];
+    if (w.dev || !create) return w.dev ? &w : nullptr;
+    std::lock_guard<std::mutex> lk(g_nf_mu);
+    if (!w.dev) {
+        void *h = nullptr, *d = nullptr;
+        if (hipHostMalloc(&h, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) return nullptr;
+        if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess) return nullptr;
+        *(volatile unsigned *)h = 0u;
+        w.host = (volatile unsigned *)h;
+        w.dev = (unsigned *)d;
+    }
+    return &w;
+}
+
+unsigned *nonfinite_word()
+{
+    NfWord *w = nf_for_current(true);
+    return w ? w->dev : nullptr;
+}
+
+unsigned *nonfinite_word_current()
+{
+    NfWord *w = nf_for_current(false);
+    return w ? w->dev : nullptr;
+}
+
+bool nonfinite_raised()
+{
+    NfWord *w = nf_for_current(false);
+    return w && *w->host != 0u;
+}
+
+// Entry check of every RHS / stage / step call (creates the device's watch
+// word on its first call there, so the kernels it launches carry it).
+static int nonfinite_entry(const char *what)
+{
+    (void)nonfinite_word();
+    if (nonfinite_raised())
+        return set_error(ACX_ENONFINITE, "%s: a kernel on this device wrote a NaN or Inf "
+                                         "(acx_nonfinite_clear resets the watch)", what);
+    return ACX_OK;
+}
+
 
 static int check_grid(const acx_grid *g)
 {
@@ -83,8 +140,6 @@ static int check_grid(const acx_grid *g)
         if (d == 2 && g->n[2] == 1 && g->bc[2] == ACX_PERIODIC) continue;
         if (g->n[d] < 4) return set_error(ACX_EINVAL, "n[%d] = %d: at least 4 cells per axis are required", d, g->n[d]);
     }
-    if (g->n[2] == 1 && !(g->gamma_hi == 1.0 && g->gamma_lo == 0.0))
-        return set_error(ACX_EUNSUPPORTED, "2D grid (n[2] = 1): the shipped linear weights only (gamma = 1, 0)");
     if ((size_t)g->n[0] * g->n[1] > 0x7fffffffu) return set_error(ACX_EINVAL, "plane too large");
     if (g->system != ACX_WAVE2 && g->system != ACX_ACOUSTIC4)
         return set_error(ACX_EINVAL, "system %d unknown", g->system);
@@ -124,6 +179,7 @@ static void fill_lin(LinArgs &a, const acx_grid *g)
     a.s = g->c0;
     a.inv48h = 1.0 / (48.0 * g->h);
     a.lapc = g->c0 * g->c0 / (48.0 * g->h * g->h);
+    a.nf = nonfinite_word_current();
 }
 
 static void fill_cw(CwArgs &a, const acx_grid *g)
@@ -143,29 +199,49 @@ static void fill_cw(CwArgs &a, const acx_grid *g)
     a.g4 = g->gamma_hi;
     a.g3c = (1.0 - g->gamma_hi) * g->gamma_lo;
     a.g3s = (1. / 6.) * (1 - g->gamma_hi) * (1 - g->gamma_lo);
+    if (g->n[2] == 1) {
+        // One plane (2D grid): the 2D CWENO4 of 2d/weno4/bubble/reconstruction.c:128-251
+        // has five quadratic stencils (central, +-x, +-y) with weights
+        // (1 - gamma_hi) gamma_lo and (1 - gamma_hi)(1 - gamma_lo)/4 (:130-140).
+        // On a z-invariant plane the 3D stencils +-z equal the central one
+        // and every z term of the indicators and of R4 vanishes (the 2D IS_R4
+        // of :154-156 is the 3D one without them), so the 3D scheme is the 2D
+        // scheme when the sided weight is the 2D one and the central weight
+        // gives up the share of the two z stencils: the three have the same
+        // indicator on one plane, so only the sum of their linear weights (the
+        // 2D central weight) enters the result, whatever its split.
+        a.g3s = 0.25 * (1 - g->gamma_hi) * (1 - g->gamma_lo);
+        a.g3c = (1.0 - g->gamma_hi) * g->gamma_lo - 2.0 * a.g3s;
+    }
     a.inv_g4 = 1.0 / g->gamma_hi;
+    a.nf = nonfinite_word_current();
 }
 
 enum { EPI_RHS = 0, EPI_AXPY = 1, EPI_COMB = 2, EPI_F32C = 3 };
 
 // One FP64 stage: OUT = a*U0 + b*Q + c*F(Q), epilogue chosen from (a, b).
+// watch: the stage's outputs feed the non-finite watch (a step watches only
+// its last stage: a NaN or Inf of an earlier stage reaches its output).
 static int stage64(const acx_grid *g, const double *Q, const double *U0, double *OUT,
-                   double a, double b, double c, int epi, hipStream_t s)
+                   double a, double b, double c, int epi, hipStream_t s, bool watch = true)
 {
     if (is_linear(g)) {
         LinArgs la;
         fill_lin(la, g);
         la.q = Q; la.u0 = U0; la.out = OUT;
         la.a = a; la.b = b; la.c = c;
+        if (!watch) la.nf = nullptr;
         return launch_linear(la, g->system, epi, false, s);
     }
     CwArgs ca;
     fill_cw(ca, g);
     ca.q = Q; ca.u0 = U0; ca.out = OUT;
     ca.a = a; ca.b = b; ca.c = c;
+    if (!watch) ca.nf = nullptr;
     return launch_cweno(ca, g->system, epi, false, false, s);
 }
 
+// One compensated FP32 stage; only the stage that ends the step is watched.
 static int stage32c(const acx_grid *g, const float *HI, const float *D, const float *LO, float *OUT,
                     float *OUT2, double b, double c, int final_stage, const uint8_t *flags, hipStream_t s)
 {
@@ -174,6 +250,7 @@ static int stage32c(const acx_grid *g, const float *HI, const float *D, const fl
         fill_lin(la, g);
         la.q = HI; la.d = D; la.lo = LO; la.out = OUT; la.out2 = OUT2;
         la.b = b; la.c = c; la.final_stage = final_stage;
+        if (!final_stage) la.nf = nullptr;
         return launch_linear(la, g->system, EPI_F32C, true, s);
     }
     CwArgs ca;
@@ -181,7 +258,23 @@ static int stage32c(const acx_grid *g, const float *HI, const float *D, const fl
     ca.q = HI; ca.d = D; ca.lo = LO; ca.out = OUT; ca.out2 = OUT2;
     ca.b = b; ca.c = c; ca.final_stage = final_stage;
     ca.flags = flags;
+    if (!final_stage) ca.nf = nullptr;
     return launch_cweno(ca, g->system, EPI_F32C, true, flags != nullptr, s);
+}
+
+// The six arrays of a compensated FP32 step: all present and pairwise
+// distinct (the 2.5D march reads HI and the increments of neighbouring tiles
+// as halo while other tiles write the outputs).
+static int check_f32c_arrays(const float *HI, const float *LO, const float *HI_OUT, const float *LO_OUT,
+                             const float *D0, const float *D1)
+{
+    const float *p[6] = {HI, LO, HI_OUT, LO_OUT, D0, D1};
+    for (int i = 0; i < 6; ++i) {
+        if (!p[i]) return set_error(ACX_EINVAL, "f32c step: null array");
+        for (int j = 0; j < i; ++j)
+            if (p[i] == p[j]) return set_error(ACX_EINVAL, "f32c step: HI, LO, HI_OUT, LO_OUT, D0, D1 must be distinct");
+    }
+    return ACX_OK;
 }
 
 } // namespace acx
@@ -207,6 +300,7 @@ int acx_rhs_f64(const acx_grid *g, const double *U, double *RHS, void *stream)
     int rc = check_grid(g);
     if (rc) return rc;
     if (!U || !RHS || U == RHS) return set_error(ACX_EINVAL, "U and RHS must be distinct device arrays");
+    if ((rc = nonfinite_entry("rhs"))) return rc;
     return stage64(g, U, nullptr, RHS, 0.0, 0.0, 1.0, EPI_RHS, (hipStream_t)stream);
 }
 
@@ -216,6 +310,7 @@ int acx_stage_f64(const acx_grid *g, const double *Q, const double *U0, double *
     int rc = check_grid(g);
     if (rc) return rc;
     if (!Q || !OUT || OUT == Q || (U0 && OUT == U0)) return set_error(ACX_EINVAL, "OUT must not alias Q or U0");
+    if ((rc = nonfinite_entry("stage"))) return rc;
     int epi;
     if (a == 0.0 && b == 0.0 && c == 1.0) epi = EPI_RHS;
     else if (a == 0.0 && b == 1.0) epi = EPI_AXPY;
@@ -232,12 +327,96 @@ int acx_ssp_rks3_step_f64(const acx_grid *g, double *U, double *W0, double *W1, 
     int rc = check_grid(g);
     if (rc) return rc;
     if (!U || !W0 || !W1 || U == W0 || U == W1 || W0 == W1) return set_error(ACX_EINVAL, "U, W0, W1 must be distinct");
+    if ((rc = nonfinite_entry("ssp_rks3_step_f64"))) return rc;
     hipStream_t s = (hipStream_t)stream;
     // TSSSPRKS3, s = 4 (n = 2, r = 2): stage times 0, 1/2, 1, 1/2.
-    if ((rc = stage64(g, U, nullptr, W0, 0.0, 1.0, dt / 2, EPI_AXPY, s))) return rc;
-    if ((rc = stage64(g, W0, nullptr, W1, 0.0, 1.0, dt / 2, EPI_AXPY, s))) return rc;
-    if ((rc = stage64(g, W1, U, W0, 1. * 2 / 3, (2 - 1.) * 1. / 3, (2 - 1.) * dt / (2 * 3), EPI_COMB, s))) return rc;
+    if ((rc = stage64(g, U, nullptr, W0, 0.0, 1.0, dt / 2, EPI_AXPY, s, false))) return rc;
+    if ((rc = stage64(g, W0, nullptr, W1, 0.0, 1.0, dt / 2, EPI_AXPY, s, false))) return rc;
+    if ((rc = stage64(g, W1, U, W0, 1. * 2 / 3, (2 - 1.) * 1. / 3, (2 - 1.) * dt / (2 * 3), EPI_COMB, s, false)))
+        return rc;
     return stage64(g, W0, nullptr, U, 0.0, 1.0, dt / 2, EPI_AXPY, s);
+}
+
+} // extern "C"
+
+namespace acx {
+
+// The four stage launches of one FP64 step, captured once per (device,
+// grid, arrays, dt) and replayed; one cached graph per device.
+struct StepGraph {
+    bool valid = false;
+    acx_grid g;
+    double *U = nullptr, *W0 = nullptr, *W1 = nullptr;
+    double dt = 0.0;
+    hipGraphExec_t exec = nullptr;
+};
+static StepGraph g_step_graph[64];
+static std::mutex g_step_graph_mu;
+
+static bool same_grid(const acx_grid &a, const acx_grid &b) { return memcmp(&a, &b, sizeof(acx_grid)) == 0; }
+
+static int step_graph_for(const acx_grid *g, double *U, double *W0, double *W1, double dt, hipGraphExec_t *out)
+{
+    int dev = 0;
+    int rc = hip_check(hipGetDevice(&dev), "step graph: device");
+    if (rc) return rc;
+    if (dev < 0 || dev >= 64) return set_error(ACX_EUNSUPPORTED, "step graph: device %d", dev);
+    StepGraph &sg = g_step_graph[dev];
+    if (sg.valid && same_grid(sg.g, *g) && sg.U == U && sg.W0 == W0 && sg.W1 == W1 && sg.dt == dt) {
+        *out = sg.exec;
+        return ACX_OK;
+    }
+    if (sg.exec) {
+        (void)hipGraphExecDestroy(sg.exec);
+        sg.exec = nullptr;
+    }
+    sg.valid = false;
+    hipStream_t cs;
+    if ((rc = hip_check(hipStreamCreate(&cs), "step graph: capture stream"))) return rc;
+    hipGraph_t graph = nullptr;
+    rc = hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed), "step graph: begin capture");
+    if (!rc) {
+        // the stage sequence of acx_ssp_rks3_step_f64
+        rc = stage64(g, U, nullptr, W0, 0.0, 1.0, dt / 2, EPI_AXPY, cs, false);
+        if (!rc) rc = stage64(g, W0, nullptr, W1, 0.0, 1.0, dt / 2, EPI_AXPY, cs, false);
+        if (!rc)
+            rc = stage64(g, W1, U, W0, 1. * 2 / 3, (2 - 1.) * 1. / 3, (2 - 1.) * dt / (2 * 3), EPI_COMB, cs, false);
+        if (!rc) rc = stage64(g, W0, nullptr, U, 0.0, 1.0, dt / 2, EPI_AXPY, cs);
+        const int re = hip_check(hipStreamEndCapture(cs, &graph), "step graph: end capture");
+        if (!rc) rc = re;
+    }
+    if (!rc) rc = hip_check(hipGraphInstantiate(&sg.exec, graph, nullptr, nullptr, 0), "step graph: instantiate");
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipStreamDestroy(cs);
+    if (rc) return rc;
+    sg.g = *g;
+    sg.U = U;
+    sg.W0 = W0;
+    sg.W1 = W1;
+    sg.dt = dt;
+    sg.valid = true;
+    *out = sg.exec;
+    return ACX_OK;
+}
+
+} // namespace acx
+
+extern "C" {
+
+int acx_ssp_rks3_steps_graph_f64(const acx_grid *g, double *U, double *W0, double *W1, double dt, int nsteps,
+                                 void *stream)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!U || !W0 || !W1 || U == W0 || U == W1 || W0 == W1) return set_error(ACX_EINVAL, "U, W0, W1 must be distinct");
+    if (nsteps < 0) return set_error(ACX_EINVAL, "nsteps must be >= 0");
+    if ((rc = nonfinite_entry("ssp_rks3_steps_graph_f64"))) return rc;   // creates the watch word before capture
+    std::lock_guard<std::mutex> lk(g_step_graph_mu);
+    hipGraphExec_t exec = nullptr;
+    if ((rc = step_graph_for(g, U, W0, W1, dt, &exec))) return rc;
+    for (int k = 0; k < nsteps; ++k)
+        if ((rc = hip_check(hipGraphLaunch(exec, (hipStream_t)stream), "step graph: launch"))) return rc;
+    return ACX_OK;
 }
 
 int acx_ssp_rks3_step_f32c(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
@@ -245,8 +424,8 @@ int acx_ssp_rks3_step_f32c(const acx_grid *g, const float *HI, const float *LO, 
 {
     int rc = check_grid(g);
     if (rc) return rc;
-    if (!HI || !LO || !HI_OUT || !LO_OUT || !D0 || !D1 || HI == HI_OUT || LO == LO_OUT)
-        return set_error(ACX_EINVAL, "f32c step: outputs must not alias the inputs");
+    if ((rc = check_f32c_arrays(HI, LO, HI_OUT, LO_OUT, D0, D1))) return rc;
+    if ((rc = nonfinite_entry("ssp_rks3_step_f32c"))) return rc;
     hipStream_t s = (hipStream_t)stream;
     // increments relative to the base state U = HI + LO:
     //   D0 = dt/2 F(U); D1 = D0 + dt/2 F(U+D0); D0 = D1/3 + dt/6 F(U+D1);
@@ -290,30 +469,24 @@ int acx_point_get_f64(const acx_grid *g, const double *X, int i, int j, int k, i
     return rc ? rc : hip_check(hipStreamSynchronize(s), "point get");
 }
 
+int acx_nonfinite_check(void *stream)
+{
+    int rc = hip_check(hipStreamSynchronize((hipStream_t)stream), "nonfinite check");
+    if (rc) return rc;
+    return nonfinite_entry("nonfinite check");
+}
+
+int acx_nonfinite_clear(void)
+{
+    NfWord *w = nf_for_current(true);
+    if (!w) return set_error(ACX_EHIP, "nonfinite clear: watch word unavailable");
+    *w->host = 0u;
+    return ACX_OK;
+}
+
 } // extern "C"
 
 namespace acx {
-
-// Tile mask of the selective step: one byte per CWENO tile, kept per device
-// and grown on demand (selective steps issued concurrently on one device
-// share it and must not overlap).
-static uint8_t *sel_mask_scratch(size_t n)
-{
-    static std::mutex mu;
-    static uint8_t *buf[64] = {};
-    static size_t cap[64] = {};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-    std::lock_guard<std::mutex> lk(mu);
-    if (cap[dev] < n) {
-        if (buf[dev]) (void)hipFree(buf[dev]);
-        buf[dev] = nullptr;
-        cap[dev] = 0;
-        if (hipMalloc(&buf[dev], n) != hipSuccess) return nullptr;
-        cap[dev] = n;
-    }
-    return buf[dev];
-}
 
 // Patch tiling: 30 x 14 x zc cells, zc read once per process, so a tile mask
 // made by acx_sel_mask stays valid for every later selective step.
@@ -334,6 +507,7 @@ static int stage32c_sel(const acx_grid *g, const float *HI, const float *D, cons
     fill_lin(la, g);
     la.q = HI; la.d = D; la.lo = LO; la.out = OUT; la.out2 = OUT2;
     la.b = b; la.c = c; la.final_stage = final_stage;
+    if (!final_stage) la.nf = nullptr;
     int rc = launch_linear(la, g->system, EPI_F32C, true, s);
     if (rc) return rc;
     CwArgs ca;
@@ -342,6 +516,7 @@ static int stage32c_sel(const acx_grid *g, const float *HI, const float *D, cons
     ca.b = b; ca.c = c; ca.final_stage = final_stage;
     ca.flags = flags;
     ca.tilemask = mask;
+    if (!final_stage) ca.nf = nullptr;
     return launch_cweno(ca, g->system, EPI_F32C, true, true, s);
 }
 
@@ -354,10 +529,9 @@ static int check_sel(const acx_grid *g, const float *HI, const float *LO, const 
     int rc = check_grid(g);
     if (rc) return rc;
     if (!flags) return set_error(ACX_EINVAL, "flags is NULL");
-    if (!HI || !LO || !HI_OUT || !LO_OUT || !D0 || !D1 || HI == HI_OUT || LO == LO_OUT)
-        return set_error(ACX_EINVAL, "f32c step: outputs must not alias the inputs");
+    if ((rc = check_f32c_arrays(HI, LO, HI_OUT, LO_OUT, D0, D1))) return rc;
     if (g->system != ACX_ACOUSTIC4) return set_error(ACX_EUNSUPPORTED, "selective step: ACOUSTIC4 only");
-    return ACX_OK;
+    return nonfinite_entry("selective step");
 }
 
 // The four selective stages given the tile mask (null mask: the check mode
@@ -388,10 +562,13 @@ extern "C" int acx_ssp_rks3_step_f32c_sel(const acx_grid *g, const float *HI, co
     if (env_int("ACX_SEL_FULL", 0)) return sel_stages(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags, nullptr, dt, s);
     CwArgs ca;
     fill_cw_patch(ca, g);
-    uint8_t *mask = sel_mask_scratch(cweno_sel_tiles(ca));
-    if (!mask) return set_error(ACX_EHIP, "selective step: tile mask allocation failed");
-    if ((rc = launch_cweno_tilemask(ca, flags, mask, s))) return rc;
-    return sel_stages(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags, mask, dt, s);
+    // the tile mask of this call, from the stream-ordered pool of s
+    uint8_t *mask = nullptr;
+    if ((rc = hip_check(hipMallocAsync((void **)&mask, cweno_sel_tiles(ca), s), "selective step: tile mask"))) return rc;
+    rc = launch_cweno_tilemask(ca, flags, mask, s);
+    if (!rc) rc = sel_stages(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags, mask, dt, s);
+    const int rf = hip_check(hipFreeAsync(mask, s), "selective step: tile mask free");
+    return rc ? rc : rf;
 }
 
 extern "C" size_t acx_sel_mask_bytes(const acx_grid *g)

@@ -127,6 +127,7 @@ k_cweno(CwArgs p)
     const int gx = gx0 + tx, gy = gy0 + ty;
     const int zb = (tile / (p.ntx * p.nty)) * p.zc, ze = min(zb + p.zc, nz);
     const bool interior = tx >= 1 && tx <= BX - 2 && ty >= 1 && ty <= BY - 2 && gx < nx && gy < ny;
+    bool nfbad = false;   // non-finite watch of this thread's outputs
     const size_t plane = (size_t)nx * ny;
     const int kx = WALLS ? p.bc[0] : 0, ky = WALLS ? p.bc[1] : 0, kz = WALLS ? p.bc[2] : 0;   // boundary kinds
 
@@ -385,6 +386,7 @@ k_cweno(CwArgs p)
                         o = nh;
                     }
                 }
+                if (p.nf) finite_acc<T, NV>(nfbad, o);
                 store_cell<T, NV>(OUT, cell, o);
             }
 #pragma unroll
@@ -394,6 +396,7 @@ k_cweno(CwArgs p)
         for (int v = 0; v < NS; ++v) { zp_prev[v] = fpz[v]; dxy_prev[v] = dxy[v]; }
         vcen_prev = vcen;
     }
+    watch_flush(p.nf, nfbad);
 }
 
 // ---------------------------------------------------------------------------

@@ -113,6 +113,24 @@ __device__ __forceinline__ void store_cell(T *__restrict__ base, size_t idx, con
     }
 }
 
+// Non-finite watch.  finite_acc marks a thread whose output cell holds a NaN
+// or +-Inf; watch_flush, called once after a tile's march by every thread of
+// the block, lets the first marked lane of each wavefront store 1 to *nf, a
+// host-mapped word (no store on the finite path).
+template <typename T, int NV>
+__device__ __forceinline__ void finite_acc(bool &bad, const Cell<T, NV> &c)
+{
+#pragma unroll
+    for (int v = 0; v < NV; ++v) bad |= !__builtin_isfinite(c.v[v]);
+}
+
+__device__ __forceinline__ void watch_flush(unsigned *nf, bool bad)
+{
+    if (!nf) return;
+    const unsigned long long m = __ballot(bad);
+    if (bad && (int)__lane_id() == __ffsll((long long)m) - 1) *(volatile unsigned *)nf = 1u;
+}
+
 // Streaming forms of load_cell / store_cell (nontemporal hint: the line is
 // not kept in L2 for reuse), for data that no other tile of the launch reads.
 typedef double acx_dbl2 __attribute__((ext_vector_type(2)));

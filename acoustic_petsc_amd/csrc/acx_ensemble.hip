@@ -155,6 +155,7 @@ static int ens_stage(acx_ensemble *e, const float *D, float *OUT, float *OUT2, d
     a.invrho = 1.0;
     a.s = 1.0;
     a.inv48h = 1.0 / (48.0 * e->h);
+    a.nf = final_stage ? nonfinite_word_current() : nullptr;   // the step's output is watched
     // bc = periodic (0) from memset
     return kConfigs[e->config].by == 8 ? ens_launch<8>(e, a, s) : ens_launch<16>(e, a, s);
 }
@@ -188,6 +189,7 @@ int acx_ensemble_create(int nprob, const int32_t *n, double h, acx_ensemble **ou
     e->ncell = 0;
     for (int p = 0; p < nprob; ++p) e->ncell += (size_t)n[3 * p] * n[3 * p + 1] * n[3 * p + 2];
     e->config = 1;   // rows 8, z-chunk 32 until tuned
+    (void)nonfinite_word();   // the watch word exists before any capture
     e->graph = nullptr;
     e->graph_stream = nullptr;
     const size_t bytes = e->ncell * 4 * sizeof(float);
@@ -288,12 +290,14 @@ int acx_ensemble_get_config(const acx_ensemble *e) { return e ? e->config : -1; 
 int acx_ensemble_step(acx_ensemble *e, double dt, void *stream)
 {
     if (!e) return set_error(ACX_EINVAL, "ensemble: null");
+    if (nonfinite_raised()) return set_error(ACX_ENONFINITE, "ensemble step: a kernel on this device wrote a NaN or Inf");
     return ens_step(e, dt, (hipStream_t)stream);
 }
 
 int acx_ensemble_graph_steps(acx_ensemble *e, double dt, int nsteps, void *stream)
 {
     if (!e || nsteps < 0 || nsteps % 2) return set_error(ACX_EINVAL, "ensemble: nsteps must be even");
+    if (nonfinite_raised()) return set_error(ACX_ENONFINITE, "ensemble steps: a kernel on this device wrote a NaN or Inf");
     hipStream_t s = (hipStream_t)stream;
     int rc;
     if (!e->graph || e->graph_dt != dt || e->graph_config != e->config || e->graph_cur != e->cur) {

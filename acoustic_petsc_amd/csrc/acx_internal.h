@@ -22,6 +22,7 @@ struct LinArgs {
     int final_stage;
     double a, b, c;    // epilogue coefficients
     double K, invrho, s, inv48h, lapc;
+    unsigned *nf;      // non-finite watch word (host-mapped; null: no watch)
 };
 
 // Arguments of the fused nonlinear CWENO4 stage kernel (acx_cweno.hip).
@@ -41,6 +42,7 @@ struct CwArgs {
     double a, b, c;
     double K, invrho, s, inv_h, c2;
     double g4, g3c, g3s, inv_g4;   // linear weights: gamma_R4, gamma_R3[0], gamma_R3[1..6], 1/gamma_R4
+    unsigned *nf;                  // non-finite watch word (host-mapped; null: no watch)
 };
 
 int launch_linear(const LinArgs &a, int sys, int epi, bool fp32, hipStream_t s);
@@ -57,5 +59,13 @@ int device_cus();
 // integer tuning knob from the environment (dflt when unset or empty)
 int env_int(const char *name, int dflt);
 int hip_check(hipError_t e, const char *what);
+// Non-finite watch of the current device: the device address of its
+// host-mapped flag word (nonfinite_word: created on first use, null if that
+// fails; nonfinite_word_current: the existing word or null, never
+// allocates), and whether a kernel has raised it (a host read, no
+// synchronisation).
+unsigned *nonfinite_word();
+unsigned *nonfinite_word_current();
+bool nonfinite_raised();
 
 } // namespace acx

@@ -132,6 +132,82 @@ __global__ void __launch_bounds__(256) k_labels(const double *__restrict__ U, Cw
     }
 }
 
+// ---- full-WENO judge of the selective step ---------------------------------------
+//
+// The full CWENO4 reconstruction (FP32, ratio-form weights, grid gammas) of
+// every cell and component of the FP32 state the selective step advances.
+// For the (cell, component) pairs a window of selective steps ran with the
+// linear polynomial (bit clear in `used`) the judge measures what that
+// choice changed: the largest difference between the CWENO4 and the linear
+// face averages over the six faces, |dc2|/6 + |dc1/2 + dc3/20| along each
+// axis (the face-average collapse of the basis).  A difference above `tol`
+// is a miss (counted); the largest difference is kept (float bits, atomic
+// max of non-negative values).  Independently, label bit v of a cell is set
+// when the eight normalised weights of component v leave the linear weights
+// by more than `thr` (the training label), and out = base | labels.
+__global__ void __launch_bounds__(256) k_judge(const float *__restrict__ HI, CwArgs p, float thr, float tol,
+                                               const uint8_t *__restrict__ used, const uint8_t *__restrict__ base,
+                                               uint8_t *__restrict__ out, unsigned long long *__restrict__ missed,
+                                               unsigned *__restrict__ worst)
+{
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const size_t ncell = (size_t)nx * ny * nz;
+    const size_t cell = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    unsigned miss = 0;
+    float wmax = 0.f;
+    if (cell < ncell) {
+        const int i = (int)(cell % nx), j = (int)((cell / nx) % ny), k = (int)(cell / ((size_t)nx * ny));
+        auto W = [](int q, int n) { return q < 0 ? q + n : (q >= n ? q - n : q); };
+        // stencil slots of cweno_weights: centre, x/y/z arms (m2 m1 p1 p2), xy/yz/zx diagonals, corners
+        const int dx[33] = {0, -2, -1, 1, 2, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, 1, -1, -1, 1, -1};
+        const int dy[33] = {0, 0, 0, 0, 0, -2, -1, 1, 2, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 0, 0, 0, 0, 1, 1, -1, 1, -1, 1, -1, -1};
+        const int dz[33] = {0, 0, 0, 0, 0, 0, 0, 0, 0, -2, -1, 1, 2, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 1, 1, 1, -1, 1, -1, -1, -1};
+        size_t off[33];
+#pragma unroll
+        for (int q = 0; q < 33; ++q)
+            off[q] = (((size_t)W(k + dz[q], nz) * ny + W(j + dy[q], ny)) * nx + W(i + dx[q], nx)) * 4;
+        const unsigned uflag = used ? used[cell] : 0xFu;
+        unsigned lab = 0;
+        for (int v = 0; v < 4; ++v) {
+            float st[33];
+#pragma unroll
+            for (int q = 0; q < 33; ++q) st[q] = HI[off[q] + v];
+            Axis3<float> o;
+            float wn[8];
+            cweno_axes<float>(st, p, o, wn);
+            float dev = fabsf(wn[0] - float(p.g4)) + fabsf(wn[1] - float(p.g3c));
+#pragma unroll
+            for (int q = 2; q < 8; ++q) dev += fabsf(wn[q] - float(p.g3s));
+            if (dev > thr) lab |= 1u << v;
+            if (!((uflag >> v) & 1u)) {
+                float c1, c2, c3, d = 0.f;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int m2 = 1 + 4 * a;   // arm slots m2 m1 p1 p2 of axis a
+                    axis_linear(st[m2], st[m2 + 1], st[0], st[m2 + 2], st[m2 + 3], c1, c2, c3);
+                    const float dd = fabsf(o.c2[a] - c2) * (1.f / 6.f) +
+                                     fabsf(0.5f * (o.c1[a] - c1) + 0.05f * (o.c3[a] - c3));
+                    d = fmaxf(d, dd);
+                }
+                wmax = fmaxf(wmax, d);
+                if (d > tol) ++miss;
+            }
+        }
+        if (out) out[cell] = (uint8_t)((base ? base[cell] : 0) | lab);
+    }
+    if (missed) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            miss += __shfl_down(miss, o, 64);
+            wmax = fmaxf(wmax, __shfl_down(wmax, o, 64));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            if (miss) atomicAdd(missed, (unsigned long long)miss);
+            if (wmax > 0.f) atomicMax(worst, __float_as_uint(wmax));   // non-negative floats order as their bits
+        }
+    }
+}
+
 // ---- MLP forward on MFMA ------------------------------------------------------
 
 // Logit z of 16 samples per wavefront (y = sigmoid(z)); the biases enter as
@@ -526,6 +602,46 @@ int acx_indicator_flags(const acx_grid *g, const float *HI, const float *D, cons
                        (const MlpPacked *)packed,
                        HI, D, g->n[0], g->n[1], g->n[2], zthr, flags);
     return hip_check(hipGetLastError(), "indicator");
+}
+
+int acx_weno_judge_f32(const acx_grid *g, const float *HI, const uint8_t *used, const uint8_t *base, double thr,
+                       double tol, uint8_t *out, long long *missed, double *worst, void *stream)
+{
+    int rc = check_ac4(g);
+    if (rc) return rc;
+    if (!HI || (!out && !missed) || (missed && !used)) return set_error(ACX_EINVAL, "judge: bad arguments");
+    CwArgs p;
+    memset(&p, 0, sizeof(p));
+    p.nx = g->n[0]; p.ny = g->n[1]; p.nz = g->n[2];
+    p.g4 = g->gamma_hi;
+    p.g3c = (1.0 - g->gamma_hi) * g->gamma_lo;
+    p.g3s = (1. / 6.) * (1 - g->gamma_hi) * (1 - g->gamma_lo);
+    p.inv_g4 = 1.0 / g->gamma_hi;
+    const size_t nc = (size_t)g->n[0] * g->n[1] * g->n[2];
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long *d = nullptr;   // [0] misses, [1] low word: worst difference (float bits)
+    if (missed) {
+        if ((rc = hip_check(hipMallocAsync((void **)&d, 2 * sizeof(unsigned long long), s), "judge counter"))) return rc;
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, (float *)d, 4, 0.f);
+    }
+    hipLaunchKernelGGL(k_judge, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, HI, p, (float)thr, (float)tol,
+                       used, base, out, d, d ? (unsigned *)(d + 1) : nullptr);
+    rc = hip_check(hipGetLastError(), "judge");
+    if (missed) {
+        unsigned long long h[2] = {0, 0};
+        if (!rc) rc = hip_check(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s), "judge copy");
+        const int rf = hip_check(hipFreeAsync(d, s), "judge counter free");
+        if (!rc) rc = rf;
+        if (!rc) rc = hip_check(hipStreamSynchronize(s), "judge sync");
+        *missed = (long long)h[0];
+        if (worst) {
+            const unsigned bits = (unsigned)(h[1] & 0xffffffffu);
+            float f;
+            memcpy(&f, &bits, sizeof(f));
+            *worst = f;
+        }
+    }
+    return rc;
 }
 
 int acx_flags_dilate(const acx_grid *g, const uint8_t *in, uint8_t *out, void *stream)

@@ -71,48 +71,22 @@ static int launch_linear_v(const LinArgs &a, hipStream_t s)
     return hip_check(hipGetLastError(), "k_linear launch");
 }
 
-// Launch variants (ACX_LIN_VARIANT): lead-plane march, tile 64 x 8, loads 1
-// plane ahead (0: 2 waves per SIMD, 1: registers capped for 4 waves per
-// SIMD), 2 planes ahead (2) or 3 planes ahead (3); tile 32 x 16 with loads 2
-// (4), 1 (5) or 3 (6) planes ahead; centre-plane march, tile 64 x 8 (7) or
-// 32 x 16 (8), centre-plane 64 x 8 capped for 4 waves per SIMD (9); centre-plane
-// 64 x 8 with late epilogue loads (10), and capped for 4 waves per SIMD (11);
-// centre-plane 16 x 32 (12); centre-plane 32 x 16 with streaming epilogue
-// loads and stores (13); centre-plane 32 x 16 with the halo ring (14), and
-// with streaming epilogue (15); lead-plane 64 x 8 with loads 1 plane ahead
-// and nontemporal stores (20); any other value: lead-plane 64 x 8 with loads
-// 2 planes ahead.
-// Defaults (measured at 256^3 and 384^3): FP64 variant 15 (the halo ring
-// with nontemporal epilogue traffic: +0.7-1.2 % over 14), FP32 variant 20
-// (variant 0 with nontemporal stores: +2.5 %).
-// Grids with a wall on any axis take the default variant of their precision
-// instantiated with ghost signs.
+// One launch shape per precision, chosen by timing every candidate at 256^3
+// and 384^3 on MI355X: FP64, the centre-plane march with the 5-plane halo
+// ring on 32 x 16 tiles, nontemporal epilogue loads and stores; FP32, the
+// lead-plane march on 64 x 8 tiles with loads one plane ahead and
+// nontemporal stores.  An all-periodic grid takes the instantiation without
+// ghost-sign work, a grid with a wall on any axis the one with it.
 template <typename T, int SYS, int EPI>
 static int launch_linear_t(const LinArgs &a, hipStream_t s)
 {
-    if (a.bc[0] != ACX_PERIODIC || a.bc[1] != ACX_PERIODIC || a.bc[2] != ACX_PERIODIC) {
-        if constexpr (sizeof(T) == 8) return launch_linear_v<T, SYS, EPI, -2, 1, 32, 16, true, true>(a, s);
-        else return launch_linear_v<T, SYS, EPI, 1, 1, 64, 8, true, true>(a, s);
-    }
-    static const int var = env_int("ACX_LIN_VARIANT", sizeof(T) == 8 ? 15 : 20);
-    switch (var) {
-    case 0: return launch_linear_v<T, SYS, EPI, 1, 1, 64, 8>(a, s);
-    case 1: return launch_linear_v<T, SYS, EPI, 1, 4, 64, 8>(a, s);
-    case 3: return launch_linear_v<T, SYS, EPI, 3, 1, 64, 8>(a, s);
-    case 4: return launch_linear_v<T, SYS, EPI, 2, 1, 32, 16>(a, s);
-    case 5: return launch_linear_v<T, SYS, EPI, 1, 1, 32, 16>(a, s);
-    case 6: return launch_linear_v<T, SYS, EPI, 3, 1, 32, 16>(a, s);
-    case 7: return launch_linear_v<T, SYS, EPI, 0, 1, 64, 8>(a, s);
-    case 8: return launch_linear_v<T, SYS, EPI, 0, 1, 32, 16>(a, s);
-    case 9: return launch_linear_v<T, SYS, EPI, 0, 4, 64, 8>(a, s);
-    case 10: return launch_linear_v<T, SYS, EPI, -1, 1, 64, 8>(a, s);
-    case 11: return launch_linear_v<T, SYS, EPI, -1, 4, 64, 8>(a, s);
-    case 12: return launch_linear_v<T, SYS, EPI, 0, 1, 16, 32>(a, s);
-    case 13: return launch_linear_v<T, SYS, EPI, 0, 1, 32, 16, false, true>(a, s);
-    case 14: return launch_linear_v<T, SYS, EPI, -2, 1, 32, 16>(a, s);
-    case 15: return launch_linear_v<T, SYS, EPI, -2, 1, 32, 16, false, true>(a, s);
-    case 20: return launch_linear_v<T, SYS, EPI, 1, 1, 64, 8, false, true>(a, s);
-    default: return launch_linear_v<T, SYS, EPI, 2, 1, 64, 8>(a, s);
+    const bool walls = a.bc[0] != ACX_PERIODIC || a.bc[1] != ACX_PERIODIC || a.bc[2] != ACX_PERIODIC;
+    if constexpr (sizeof(T) == 8) {
+        return walls ? launch_linear_v<T, SYS, EPI, -2, 1, 32, 16, true, true>(a, s)
+                     : launch_linear_v<T, SYS, EPI, -2, 1, 32, 16, false, true>(a, s);
+    } else {
+        return walls ? launch_linear_v<T, SYS, EPI, 1, 1, 64, 8, true, true>(a, s)
+                     : launch_linear_v<T, SYS, EPI, 1, 1, 64, 8, false, true>(a, s);
     }
 }
 

@@ -157,6 +157,7 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
 #pragma unroll
     for (int u = 0; u < PF; ++u) issue(sl[u], jbeg + u);
 
+    bool nfbad = false;   // non-finite watch of this thread's outputs
     int buf = 0;
     for (int jb = jbeg; jb <= jend; jb += PF) {
 #pragma unroll
@@ -303,12 +304,14 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
                         o = nh;
                     }
                 }
+                if (p.nf) finite_acc<T, NV>(nfbad, o);
                 if constexpr (NT) store_cell_nt<T, NV>(OUT, cidx, o);
                 else store_cell<T, NV>(OUT, cidx, o);
             }
             issue(S, j + PF);   // refill this slot for plane j + PF
         }
     }
+    watch_flush(p.nf, nfbad);
 }
 
 // Centre-plane march of the same tile: the own column's z-queue holds planes
@@ -438,6 +441,7 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
     const T inv48h = T(p.inv48h), Kc = T(p.K), irho = T(p.invrho), sLLF = T(p.s);
     const T lap = T(p.lapc);
 
+    bool nfbad = false;               // non-finite watch of this thread's outputs
     int buf = 0;                      // slot of plane z
     int hslot = HRING ? 4 : 0;        // ring slot of plane z + 4
     for (int z = zb; z < ze; ++z) {
@@ -539,6 +543,7 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
                     o = nh;
                 }
             }
+            if (p.nf) finite_acc<T, NV>(nfbad, o);
             if constexpr (NT) store_cell_nt<T, NV>(OUT, idx, o);
             else store_cell<T, NV>(OUT, idx, o);
         }
@@ -561,6 +566,7 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
             buf ^= 1;
         }
     }
+    watch_flush(p.nf, nfbad);
 }
 
 } // namespace acx

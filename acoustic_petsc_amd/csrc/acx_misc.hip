@@ -15,12 +15,16 @@ namespace acx {
 constexpr int RED_THREADS = 256;
 constexpr int RED_BLOCKS = 1024;
 
+// max that keeps a NaN (fmax drops it): a NaN anywhere makes the result NaN,
+// as PETSc's VecNorm(NORM_INFINITY) reports it
+__device__ __forceinline__ double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
+
 __device__ __forceinline__ void wave_reduce(double &s, double &m)
 {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         s += __shfl_down(s, off, 64);
-        m = fmax(m, __shfl_down(m, off, 64));
+        m = nmax(m, __shfl_down(m, off, 64));
     }
 }
 
@@ -47,7 +51,7 @@ k_reduce_pass1(const double *__restrict__ X, size_t n, int nvar, int comp, doubl
         if (comp >= 0 && (int)(i % nvar) != comp) continue;
         double v = X[i];
         s += v * v;
-        m = fmax(m, fabs(v));
+        m = nmax(m, fabs(v));
     }
     block_reduce(s, m);
     if (threadIdx.x == 0) { part[2 * blockIdx.x] = s; part[2 * blockIdx.x + 1] = m; }
@@ -62,7 +66,7 @@ k_diff_pass1(const float *__restrict__ HI, const float *__restrict__ LO, const d
         double a = (double)HI[i] + (LO ? (double)LO[i] : 0.0);
         double d = a - B[i];
         s += d * d;
-        m = fmax(m, fabs(d));
+        m = nmax(m, fabs(d));
         sb += B[i] * B[i];
     }
     block_reduce(s, m);
@@ -80,7 +84,7 @@ k_energy_pass1(const double *__restrict__ U, size_t ncell, double invK, double r
         double2 a = p[0], b = p[1];
         double e = a.x * a.x * invK + rho0 * (a.y * a.y + b.x * b.x + b.y * b.y);
         s += e;
-        m = fmax(m, e);
+        m = nmax(m, e);
     }
     block_reduce(s, m);
     if (threadIdx.x == 0) { part[2 * blockIdx.x] = s; part[2 * blockIdx.x + 1] = m; }
@@ -93,7 +97,7 @@ k_reduce_pass2(const double *__restrict__ part, int nblk, int stride, double *__
     double s = 0.0, m = 0.0, s2 = 0.0, dummy = 0.0;
     for (int i = threadIdx.x; i < nblk; i += RED_THREADS) {
         s += part[stride * i];
-        m = fmax(m, part[stride * i + 1]);
+        m = nmax(m, part[stride * i + 1]);
         if (stride > 2) s2 += part[stride * i + 2];
     }
     block_reduce(s, m);
@@ -102,18 +106,15 @@ k_reduce_pass2(const double *__restrict__ part, int nblk, int stride, double *__
     if (threadIdx.x == 0) { out[0] = s; out[1] = m; out[2] = s2; }
 }
 
-// Partials + result buffer, allocated once per device on first use.
-static double *scratch_for_device()
+// Partials (3 per block) + result of one reduction.
+constexpr size_t RED_SCRATCH = 3 * RED_BLOCKS + 4;
+
+// The partials of one call, taken from the stream-ordered pool of `s` and
+// returned to it when the call ends, so calls on different streams never
+// share them.
+static int red_alloc(double **part, hipStream_t s)
 {
-    static double *buf[64] = {nullptr};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-    if (!buf[dev]) {
-        double *p = nullptr;
-        if (hipMalloc(&p, (3 * RED_BLOCKS + 4) * sizeof(double)) != hipSuccess) return nullptr;
-        buf[dev] = p;
-    }
-    return buf[dev];
+    return hip_check(hipMallocAsync((void **)part, RED_SCRATCH * sizeof(double), s), "reduce scratch");
 }
 
 static int finish(double *part, int stride, double *host_out, int nout, hipStream_t s)
@@ -121,11 +122,11 @@ static int finish(double *part, int stride, double *host_out, int nout, hipStrea
     double *res = part + 3 * RED_BLOCKS;
     hipLaunchKernelGGL(k_reduce_pass2, dim3(1), dim3(RED_THREADS), 0, s, part, RED_BLOCKS, stride, res);
     int rc = hip_check(hipGetLastError(), "reduce pass 2");
-    if (rc) return rc;
     double tmp[3];
-    rc = hip_check(hipMemcpyAsync(tmp, res, 3 * sizeof(double), hipMemcpyDeviceToHost, s), "reduce copy");
-    if (rc) return rc;
-    rc = hip_check(hipStreamSynchronize(s), "reduce sync");
+    if (!rc) rc = hip_check(hipMemcpyAsync(tmp, res, 3 * sizeof(double), hipMemcpyDeviceToHost, s), "reduce copy");
+    const int rf = hip_check(hipFreeAsync(part, s), "reduce scratch free");
+    if (!rc) rc = rf;
+    if (!rc) rc = hip_check(hipStreamSynchronize(s), "reduce sync");
     if (rc) return rc;
     memcpy(host_out, tmp, nout * sizeof(double));
     return ACX_OK;
@@ -170,7 +171,7 @@ k_ctcs(const double *__restrict__ Pold, const double *__restrict__ P, double *__
     const double cc = C[o], dt = c.dt;
     double pn = 2.0 * p0 - Pold[o] + cc * cc * dt * dt * (pxx + pyy + pzz);
     if (i == c.src[0] && j == c.src[1] && k == c.src[2]) pn += ctcs_source(it * dt, c.f0) / (c.dx * c.dy * c.dz) * dt * dt;
-    if (i == c.rcv[0] && j == c.rcv[1] && k == c.rcv[2]) rcv[0] = p0;
+    if (rcv && i == c.rcv[0] && j == c.rcv[1] && k == c.rcv[2]) rcv[0] = p0;
     Pnew[o] = pn;
 }
 
@@ -254,7 +255,7 @@ k_ctcs_z(const double *__restrict__ Pold, const double *__restrict__ P, double *
             double pn = 2.0 * p0 - po + cc * cc * dt * dt * (pxx + pyy + pzz);
             if (i == c.src[0] && j == c.src[1] && z == c.src[2])
                 pn += ctcs_source(it * dt, c.f0) / (c.dx * c.dy * c.dz) * dt * dt;
-            if (i == c.rcv[0] && j == c.rcv[1] && z == c.rcv[2]) rcv[0] = p0;
+            if (rcv && i == c.rcv[0] && j == c.rcv[1] && z == c.rcv[2]) rcv[0] = p0;
             Pnew[o] = pn;
         }
 #pragma unroll
@@ -297,12 +298,15 @@ extern "C" {
 int acx_norms_f64(const double *X, size_t ncells, int nvar, int comp, double *out, void *stream)
 {
     if (!X || !out || nvar < 1 || comp >= nvar) return set_error(ACX_EINVAL, "norms: bad arguments");
-    double *part = scratch_for_device();
-    if (!part) return set_error(ACX_EHIP, "norms: scratch allocation failed");
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_reduce_pass1, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, X, ncells * nvar, nvar, comp, part);
-    int rc = hip_check(hipGetLastError(), "norms pass 1");
+    double *part = nullptr;
+    int rc = red_alloc(&part, s);
     if (rc) return rc;
+    hipLaunchKernelGGL(k_reduce_pass1, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, X, ncells * nvar, nvar, comp, part);
+    if ((rc = hip_check(hipGetLastError(), "norms pass 1"))) {
+        (void)hipFreeAsync(part, s);
+        return rc;
+    }
     double r[3];
     if ((rc = finish(part, 2, r, 3, s))) return rc;
     out[0] = r[0];
@@ -313,12 +317,15 @@ int acx_norms_f64(const double *X, size_t ncells, int nvar, int comp, double *ou
 int acx_diff_f32c_f64(const float *HI, const float *LO, const double *B, size_t n, double *out, void *stream)
 {
     if (!HI || !B || !out) return set_error(ACX_EINVAL, "diff: bad arguments");
-    double *part = scratch_for_device();
-    if (!part) return set_error(ACX_EHIP, "diff: scratch allocation failed");
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_diff_pass1, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, HI, LO, B, n, part);
-    int rc = hip_check(hipGetLastError(), "diff pass 1");
+    double *part = nullptr;
+    int rc = red_alloc(&part, s);
     if (rc) return rc;
+    hipLaunchKernelGGL(k_diff_pass1, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, HI, LO, B, n, part);
+    if ((rc = hip_check(hipGetLastError(), "diff pass 1"))) {
+        (void)hipFreeAsync(part, s);
+        return rc;
+    }
     double r[3];
     if ((rc = finish(part, 3, r, 3, s))) return rc;
     out[0] = r[2] > 0 ? sqrt(r[0] / r[2]) : sqrt(r[0]);
@@ -329,14 +336,17 @@ int acx_diff_f32c_f64(const float *HI, const float *LO, const double *B, size_t 
 int acx_energy_f64(const acx_grid *g, const double *U, double *out, void *stream)
 {
     if (!g || !U || !out || g->system != ACX_ACOUSTIC4) return set_error(ACX_EINVAL, "energy: ACOUSTIC4 state required");
-    double *part = scratch_for_device();
-    if (!part) return set_error(ACX_EHIP, "energy: scratch allocation failed");
     hipStream_t s = (hipStream_t)stream;
+    double *part = nullptr;
+    int rc = red_alloc(&part, s);
+    if (rc) return rc;
     const size_t nc = (size_t)g->n[0] * g->n[1] * g->n[2];
     const double K = g->rho0 * g->c0 * g->c0;
     hipLaunchKernelGGL(k_energy_pass1, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, U, nc, 1.0 / K, g->rho0, part);
-    int rc = hip_check(hipGetLastError(), "energy pass 1");
-    if (rc) return rc;
+    if ((rc = hip_check(hipGetLastError(), "energy pass 1"))) {
+        (void)hipFreeAsync(part, s);
+        return rc;
+    }
     double r[3];
     if ((rc = finish(part, 2, r, 3, s))) return rc;
     out[0] = 0.5 * r[0] * g->h * g->h * (g->n[2] == 1 ? 1.0 : g->h);   // cell area on a 2D grid
@@ -350,10 +360,14 @@ int acx_ctcs_update_f64(const acx_ctcs *c, const double *POLD, const double *P, 
     for (int d = 0; d < 3; ++d)
         if (c->n[d] < 5 && c->n[d] != 1) return set_error(ACX_EINVAL, "ctcs: at least 5 cells per axis (or 1)");
     if (c->stencil != 1 && c->stencil != 2) return set_error(ACX_EINVAL, "stencil width can be either 1 or 2");
-    double *part = scratch_for_device();
-    if (!part) return set_error(ACX_EHIP, "ctcs: scratch allocation failed");
     hipStream_t s = (hipStream_t)stream;
-    double *rcv = part + 3 * RED_BLOCKS + 3;
+    // the receiver sample goes through a one-value buffer of this call (none
+    // when the caller does not ask for it)
+    double *rcv = nullptr;
+    if (rcv_out) {
+        const int ra = hip_check(hipMallocAsync((void **)&rcv, sizeof(double), s), "ctcs receiver buffer");
+        if (ra) return ra;
+    }
     if (c->n[0] >= 5 && c->n[1] >= 5 && c->n[2] >= 5) {
         // 3D: z-march; 32 planes per block, halved (down to 4) below 1024 blocks
         const size_t cols = (size_t)((c->n[0] + CT_BX - 1) / CT_BX) * ((c->n[1] + CT_BY - 1) / CT_BY);
@@ -370,12 +384,13 @@ int acx_ctcs_update_f64(const acx_ctcs *c, const double *POLD, const double *P, 
         else hipLaunchKernelGGL(k_ctcs<2>, grd, blk, 0, s, POLD, P, PNEW, C, *c, it, rcv);
     }
     int rc = hip_check(hipGetLastError(), "ctcs launch");
-    if (rc) return rc;
     if (rcv_out) {
-        if ((rc = hip_check(hipMemcpyAsync(rcv_out, rcv, sizeof(double), hipMemcpyDeviceToHost, s), "ctcs rcv copy"))) return rc;
-        if ((rc = hip_check(hipStreamSynchronize(s), "ctcs sync"))) return rc;
+        if (!rc) rc = hip_check(hipMemcpyAsync(rcv_out, rcv, sizeof(double), hipMemcpyDeviceToHost, s), "ctcs rcv copy");
+        const int rf = hip_check(hipFreeAsync(rcv, s), "ctcs receiver buffer free");
+        if (!rc) rc = rf;
+        if (!rc) rc = hip_check(hipStreamSynchronize(s), "ctcs sync");
     }
-    return ACX_OK;
+    return rc;
 }
 
 int acx_field_gaussians(const acx_grid *g, uint64_t seed, int npulse, double amin, double amax, double smin,

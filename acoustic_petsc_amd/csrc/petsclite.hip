@@ -980,6 +980,10 @@ extern "C" PetscErrorCode TSSolve(TS ts, Vec u)
             else break;
         }
     }
+    // a NaN or Inf written by any stage of the solve (the reference aborts on
+    // an inadmissible state, 3d/weno3/LinearisedEuler/pde.c:50-55)
+    const int rc = acx_nonfinite_check(nullptr);
+    if (rc) return PetscLiteError(76 + rc, "TSSolve: %s", acx_last_error());
     return 0;
 }
 

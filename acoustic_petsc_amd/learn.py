@@ -38,6 +38,7 @@ def _bind():
         "acx_indicator_flags": [gp, vp, vp, vp, f32, vp, vp],
         "acx_flags_dilate": [gp, vp, vp, vp],
         "acx_flags_count": [vp, _C.c_size_t, _C.POINTER(_C.c_longlong), vp],
+        "acx_weno_judge_f32": [gp, vp, vp, vp, dbl, dbl, vp, vp, vp, vp],
     }.items():
         fn = getattr(L, name)
         fn.argtypes = args
@@ -216,47 +217,139 @@ def evaluate(packed, X, T, thr):
     return tp / max(int(t.sum()), 1), tp / max(int(pred.sum()), 1), float(pred.float().mean())
 
 
+def judge(grid, hi, used=None, base=None, out=None, thr=0.05, tol=0.0, stream=None):
+    """Full-WENO judge (acx_weno_judge_f32) of the FP32 state `hi`: returns
+    (out = base | labels, misses, worst).  With `used` (the flags a run of
+    selective steps ran with), misses counts the (cell, component) pairs run
+    with the linear polynomial whose CWENO4 face averages differ from the
+    linear ones by more than `tol`, and worst is the largest such difference;
+    without it both are None."""
+    import torch
+    L = _bind()
+    g, gp = _g(grid)
+    if out is None:
+        out = torch.empty(grid.ncells, dtype=torch.uint8, device="cuda")
+    missed, worst = _C.c_longlong(0), _C.c_double(0.0)
+    vp = lambda t: _C.c_void_p(t.data_ptr()) if t is not None else None
+    judged = used is not None
+    _check(L.acx_weno_judge_f32(gp, _ptr(hi, torch.float32), vp(used), vp(base), thr, tol, vp(out),
+                                _C.byref(missed) if judged else None, _C.byref(worst) if judged else None,
+                                _stream(stream)))
+    return out, (missed.value if judged else None), (worst.value if judged else None)
+
+
 class SelectiveStepper:
     """FP32 compensated SSP-RK3 with the learned indicator: flags from the
     MLP every `refresh` steps, dilated by `dilate` cells and spread over the
     four components (a cell flagged for any component is flagged for all),
     full CWENO4 where set.
 
+    Full WENO judges every window of `refresh` steps: at the next refresh the
+    full CWENO4 reconstruction of the state reached is compared, at every
+    (cell, component) the window ran with the linear polynomial, with the
+    linear one (acx_weno_judge_f32: largest face-average difference).  A
+    difference above judge_rel x max|U| (max over the initial state) is a
+    miss: the window is run again from its saved start state with full CWENO4
+    everywhere (the fallback).  The counts are kept in `misses`,
+    `redone_windows` and `judged_windows`, the largest difference seen in
+    `worst_skipped`.  The new flags are the MLP's united with the cells whose
+    weights leave the linear weights now (the training label), before the
+    dilation.  judge=False switches the check off.
+
     Defaults: at CFL 0.4 a front moves 0.4/3 cell per step, so flags taken
     every 12 steps and dilated by 3 cells still cover it (1.6 cells) with a
-    margin of more than a cell; the indicator then runs once per 12 steps."""
+    margin of more than a cell; the indicator and the judge then run once per
+    12 steps."""
 
-    def __init__(self, grid, packed, U64, thr=0.3, refresh=12, dilate=3):
+    def __init__(self, grid, packed, U64, thr=0.3, refresh=12, dilate=3, judge=True, label_thr=0.05,
+                 judge_rel=1e-7):
         import torch
         self.grid, self.packed, self.thr, self.refresh, self.dilate = grid, packed, thr, refresh, dilate
+        self.use_judge, self.label_thr = judge, label_thr
+        self.judge_tol = judge_rel * float(U64.abs().max())
+        self.worst_skipped = 0.0
         self.state = CompensatedState(grid, U64)
         self.flags = torch.empty(grid.ncells, dtype=torch.uint8, device="cuda")
         self.tmp = torch.empty_like(self.flags)
+        self.tmp2 = torch.empty_like(self.flags)
         self.mask = None
         self.nsteps = 0
         self.flag_bits = []
+        self.misses = 0
+        self.redone_windows = 0
+        self.judged_windows = 0
+        self._saved = None       # (hi, lo) at the start of the current window
+        self._window_dts = []
 
-    def refresh_flags(self, stream=None):
+    def _mlp_flags(self, stream=None):
         L = _bind()
         g, gp = _g(self.grid)
         import torch
         _check(L.acx_indicator_flags(gp, _ptr(self.state.hi, torch.float32), None,
                                      _C.c_void_p(self.packed.data_ptr()), self.thr,
                                      _C.c_void_p(self.tmp.data_ptr()), _stream(stream)))
-        src, dst = self.tmp, self.flags
+
+    def _dilate_into_flags(self, src, stream=None):
+        L = _bind()
+        g, gp = _g(self.grid)
+        spare = self.tmp if src is not self.tmp else self.tmp2
+        dst = self.flags
         for _ in range(self.dilate):
             _check(L.acx_flags_dilate(gp, _C.c_void_p(src.data_ptr()), _C.c_void_p(dst.data_ptr()), _stream(stream)))
-            src, dst = dst, src
+            src, dst = dst, (spare if dst is self.flags else self.flags)
         if src is not self.flags:
             self.flags.copy_(src)
+
+    def refresh_flags(self, stream=None):
+        """Judge the finished window (rerun it with full WENO on a miss), then
+        take new flags from the MLP and the current weights."""
+        have_window = self._saved is not None and self._window_dts
+        if self.use_judge and have_window:
+            self._mlp_flags(stream)
+            _, missed, worst = judge(self.grid, self.state.hi, used=self.flags, base=self.tmp, out=self.tmp2,
+                                     thr=self.label_thr, tol=self.judge_tol, stream=stream)
+            self.judged_windows += 1
+            self.worst_skipped = max(self.worst_skipped, worst)
+            if missed:
+                self.misses += missed
+                self.redone_windows += 1
+                self.state.hi.copy_(self._saved[0])
+                self.state.lo.copy_(self._saved[1])
+                for dt in self._window_dts:
+                    self.state.step(dt, stream=stream)      # full CWENO4 everywhere
+                self._mlp_flags(stream)
+                judge(self.grid, self.state.hi, base=self.tmp, out=self.tmp2, thr=self.label_thr, stream=stream)
+            src = self.tmp2
+        else:
+            self._mlp_flags(stream)
+            if self.use_judge:
+                judge(self.grid, self.state.hi, base=self.tmp, out=self.tmp2, thr=self.label_thr, stream=stream)
+                src = self.tmp2
+            else:
+                src = self.tmp
+        self._dilate_into_flags(src, stream)
         # the patch tiles of these flags, kept until the next refresh
         self.mask = sel_mask(self.grid, self.flags, self.mask, stream)
+        if self.use_judge:
+            if self._saved is None:
+                self._saved = (self.state.hi.clone(), self.state.lo.clone())
+            else:
+                self._saved[0].copy_(self.state.hi)
+                self._saved[1].copy_(self.state.lo)
+        self._window_dts = []
 
     def step(self, dt):
         if self.nsteps % self.refresh == 0:
             self.refresh_flags()
         self.state.step(dt, flags=self.flags, mask=self.mask)
+        self._window_dts.append(dt)
         self.nsteps += 1
+
+    def finish(self):
+        """Judge the last (partial) window; returns the total misses."""
+        if self.use_judge and self._window_dts:
+            self.refresh_flags()
+        return self.misses
 
     def flagged_fraction(self):
         L = _bind()

@@ -223,8 +223,12 @@ def variant_2d(torch, acx, steps=200, N=256):
     work = (torch.empty_like(U), torch.empty_like(U))
     dt = acx.cfl_dt(g, 0.4)
     t, _ = time_steps(torch, lambda: acx.ssp_rks3_step(g, U, dt, work), steps, 2)
-    return {"value": N * N * STAGES * steps / t, "ms_per_step": 1e3 * t / steps, "steps": steps,
-            "grid": [N, N, 1], "note": "launch-bound: 65536 cells per stage launch"}
+    # the same steps replayed from the captured hipGraph of one step
+    tg, _ = time_steps(torch, lambda: acx.ssp_rks3_steps_graph(g, U, dt, 1, work), steps, 2)
+    return {"value": N * N * STAGES * steps / tg, "ms_per_step": 1e3 * tg / steps, "steps": steps,
+            "grid": [N, N, 1], "stepper": "hipGraph replay of the four stage launches",
+            "eager_value": N * N * STAGES * steps / t, "eager_ms_per_step": 1e3 * t / steps,
+            "note": "launch-bound: 65536 cells per stage launch"}
 
 
 def variant_wave2(torch, acx, N=256, steps=20):
@@ -372,6 +376,17 @@ def variant_learned(torch, acx, sizes, steps, var, fp64_value):
         tsel, _ = time_steps(torch, lambda: sel.step(dt), k, 0)
         fr.append(sel.flagged_fraction())
         rel, mx = acx.diff_f32c(sel.state.hi, sel.state.lo, ref)
+        # cost of one full-WENO judge pass and of one indicator pass on this grid
+        t_judge, _ = time_steps(torch, lambda: learn.judge(g, sel.state.hi, used=sel.flags, base=sel.tmp,
+                                                           out=sel.tmp2, tol=sel.judge_tol), 3, 1)
+        t_ind, _ = time_steps(torch, lambda: sel._mlp_flags(), 3, 1)
+        judge_info = {"windows_judged": sel.judged_windows, "misses": sel.misses,
+                      "windows_rerun": sel.redone_windows, "judge_ms": 1e3 * t_judge / 3,
+                      "indicator_ms": 1e3 * t_ind / 3, "tolerance": sel.judge_tol,
+                      "largest_skipped_face_difference": sel.worst_skipped,
+                      "rule": "every 12-step window: at each (cell, component) run with the linear polynomial, "
+                              "full CWENO4 vs linear face averages of the state reached; a difference above "
+                              "1e-7 max|U| reruns the window with full CWENO4"}
         del sel
         lin = acx.CompensatedState(glin, U0.cuda())
         for _ in range(k + 1):
@@ -389,6 +404,7 @@ def variant_learned(torch, acx, sizes, steps, var, fp64_value):
         res[f"fp32_learned_{M}"] = {
             "value": v, "ms_per_step": 1e3 * tsel / k, "steps": k + 1, "grid": [M, M, M],
             "indicator": {"threshold": 0.3, "refresh_steps": 12, "dilation_cells": 3, "label": "sum|w-gamma| > 0.05"},
+            "judge": judge_info,
             "flagged_frac_last_step": fr[-1], "skipped_frac_last_step": 1.0 - fr[-1],
             "rel_l2_vs_fp64_weno": rel, "max_abs_vs_fp64_weno": mx,
             "linear_only_rel_l2_vs_fp64_weno": rel_lin,

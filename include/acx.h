@@ -41,10 +41,12 @@ extern "C" {
 #define ACX_TRANSMISSIVE  3   /* ACOUSTIC4: copy of the boundary cell (same file) */
 
 /* 2D grids: n[2] = 1 with bc[2] = ACX_PERIODIC is one plane, the DMDA of the
- * reference's 2D programs.  With the shipped linear weights the operator on
- * it equals the 2D WENO4 operator (2d/weno4/standing_wave/rhs_function.c:12-257
- * for ACX_WAVE2; for ACX_ACOUSTIC4 with w = 0 the 2D linearised acoustics
- * (p', u, v), w staying 0); other weights are rejected on 2D grids. */
+ * reference's 2D programs.  The operator on it is the 2D WENO4 operator
+ * (2d/weno4/standing_wave/rhs_function.c:12-257 for ACX_WAVE2; for
+ * ACX_ACOUSTIC4 with w = 0 the 2D linearised acoustics (p', u, v), w staying
+ * 0); with other than the shipped linear weights the reconstruction is the 2D
+ * CWENO4 of 2d/weno4/bubble/reconstruction.c:128-251 (five quadratic
+ * stencils). */
 typedef struct acx_grid {
     int32_t n[3];        /* cells along x, y, z (every axis >= 4, or n[2] = 1) */
     int32_t system;      /* ACX_WAVE2 or ACX_ACOUSTIC4                         */
@@ -77,6 +79,15 @@ int acx_stage_f64(const acx_grid *g, const double *Q, const double *U0, double *
  * W0, W1 are device work vectors of the same size as U. */
 int acx_ssp_rks3_step_f64(const acx_grid *g, double *U, double *W0, double *W1, double dt, void *stream);
 
+/* nsteps steps of acx_ssp_rks3_step_f64 replayed from a hipGraph of the four
+ * stage launches (the reference's stage loop, hype.c:180-183, PETSc's
+ * TSStep_SSP), captured on the first call for a (device, grid, U, W0, W1,
+ * dt) and reused while they stay the same (one cached graph per device).
+ * For small grids, where the launch overhead is a large part of a stage.
+ * Results equal acx_ssp_rks3_step_f64 bit for bit. */
+int acx_ssp_rks3_steps_graph_f64(const acx_grid *g, double *U, double *W0, double *W1, double dt, int nsteps,
+                                 void *stream);
+
 /* ---- FP32 storage with compensated step accumulation ------------------- *
  * The state is kept as a float pair (hi, lo) with hi + lo the value; stage
  * intermediates are FP32 increments D_k = q_k - (hi + lo).  One step of the
@@ -90,7 +101,10 @@ int acx_ssp_rks3_step_f32c(const acx_grid *g, const float *HI, const float *LO, 
 /* ---- reductions (FP64 accumulation) ---------------------------------- */
 
 /* out[0] = sum of squares of component `comp` (all components if comp < 0),
- * out[1] = max abs; host pointer out[2].  VecNorm (error_norms.c:87-89). */
+ * out[1] = max abs; host pointer out[2].  VecNorm (error_norms.c:87-89).
+ * A NaN anywhere makes both results NaN (as PETSc's VecNorm).  The partial
+ * sums live in a buffer taken per call on `stream` (stream-ordered), so
+ * calls on different streams do not share scratch. */
 int acx_norms_f64(const double *X, size_t ncells, int nvar, int comp, double *out, void *stream);
 /* relative L2 of (A-B) against B and max |A-B|, FP64 accumulation; A is
  * float hi (+ lo, may be NULL), B double. */
@@ -146,6 +160,20 @@ int acx_point_add_f64(const acx_grid *g, double *X, int i, int j, int k, int com
 /* *out = X[cell (i, j, k), component comp] (host result; synchronises the
  * stream), the receiver read of 3d/weno4/point_source/rhs_function.c:342-346. */
 int acx_point_get_f64(const acx_grid *g, const double *X, int i, int j, int k, int comp, double *out, void *stream);
+
+/* ---- non-finite watch (ACX_ENONFINITE) -------------------------------- *
+ * The reference aborts on an inadmissible state (3d/weno3/LinearisedEuler/
+ * pde.c:50-55, MPI_Abort when p < 1e-12); here every stage kernel raises a
+ * per-device flag when it writes a NaN or +-Inf.  The flag is read, without
+ * synchronising, at the entry of every RHS / stage / step call (acx_rhs_f64,
+ * acx_stage_f64, the steppers, the selective and ensemble steps), which then
+ * return ACX_ENONFINITE without launching: a blow-up is reported by the first
+ * such call issued after the kernel that produced it has finished.
+ * acx_nonfinite_check synchronises `stream` first and so reports everything
+ * queued on it before the call.  The flag stays raised (for every stream and
+ * state on the device) until acx_nonfinite_clear. */
+int acx_nonfinite_check(void *stream);
+int acx_nonfinite_clear(void);
 
 const char *acx_last_error(void);
 int acx_version(void);

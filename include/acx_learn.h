@@ -68,6 +68,22 @@ int acx_mlp_adam(float *params, const float *grad, float *m, float *v, int step,
 int acx_indicator_flags(const acx_grid *g, const float *HI, const float *D, const void *packed, float thr,
                         uint8_t *flags, void *stream);
 
+/* Full-WENO judge of the selective step.  Reconstructs every (cell,
+ * component) of the FP32 state HI with the full CWENO4 (grid gammas, FP32
+ * ratio-form weights).  Where `used` -- the flags a run of selective steps
+ * ran with -- has the component's bit clear, that run used the linear
+ * polynomial: the judge measures the largest difference between the CWENO4
+ * and the linear face averages of the cell (six faces) and counts a miss
+ * when it exceeds tol (absolute).  *missed (host; synchronises the stream)
+ * receives the number of misses, *worst (may be NULL) the largest
+ * difference.  Independently, label bit v of out[cell] is set when the eight
+ * normalised weights leave the linear weights, sum_k |w_k - gamma_k| > thr
+ * (the training label of acx_weno_labels_f64), and out[cell] = base[cell] |
+ * labels (base may be NULL).  out or missed may be NULL, not both; missed
+ * needs used. */
+int acx_weno_judge_f32(const acx_grid *g, const float *HI, const uint8_t *used, const uint8_t *base, double thr,
+                       double tol, uint8_t *out, long long *missed, double *worst, void *stream);
+
 /* flags_out[cell] = all four component bits (0xF) if any bit of flags_in is
  * set over the cell and its 6 face neighbours, else 0.  The components are
  * spread together because the system couples them: within one step a steep
@@ -84,8 +100,8 @@ int acx_flags_count(const uint8_t *flags, size_t ncells, long long *bits, void *
  * kernel over every cell, then the CWENO kernel over the tiles (30 x 14 x 4
  * cells; ACX_ZC_PATCH sets the 4, read once per process) that hold a flagged
  * cell or a face neighbour of one.  The tile mask is made from the flags on
- * every call, into library scratch kept per device: selective steps on one
- * device must not run concurrently.  ACOUSTIC4. */
+ * every call, into a buffer taken from the stream-ordered pool of `stream`
+ * for the call.  ACOUSTIC4. */
 int acx_ssp_rks3_step_f32c_sel(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
                                float *LO_OUT, float *D0, float *D1, const uint8_t *flags, double dt, void *stream);
 

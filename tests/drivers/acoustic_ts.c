@@ -8,8 +8,11 @@
  *   mode generic : TSSolve with a user RHS that calls RHSFunction (the
  *                  VecAXPY / VecAXPBYPCZ sequence of TSSSP RKS3)
  *   mode rk104   : generic path with TSSSPRK104
+ * An optional last argument `nan` or `inf` puts that value into the pressure
+ * of one cell of the initial state: TSSolve must then fail with the
+ * non-finite-state error instead of returning a blown-up solution.
  *
- *   usage: acoustic_ts N steps gamma_hi gamma_lo mode OUT
+ *   usage: acoustic_ts N steps gamma_hi gamma_lo mode OUT [nan|inf]
  */
 #define nVar 4
 #include <math.h>
@@ -23,7 +26,8 @@ static PetscErrorCode user_rhs(TS ts, PetscReal t, Vec U, Vec F, void *ctx)
     return RHSFunction(ts, t, U, F, ctx);
 }
 
-static PetscErrorCode run(int N, int steps, double ghi, double glo, const char *mode, const char *out)
+static PetscErrorCode run(int N, int steps, double ghi, double glo, const char *mode, const char *out,
+                          const char *seed_bad)
 {
     AppCtx c;
     DM da;
@@ -61,6 +65,7 @@ static PetscErrorCode run(int N, int steps, double ghi, double glo, const char *
             for (int j = 0; j < N; ++j)
                 for (int i = 0; i < N; ++i)
                     for (int v = 0; v < 4; ++v) u[k][j][i].comp[v] = host[(((size_t)k * N + j) * N + i) * 4 + v];
+        if (seed_bad) u[N / 3][N / 2][N / 4].comp[0] = strcmp(seed_bad, "inf") == 0 ? INFINITY : NAN;
         CHKERRQ(DMDAVecRestoreArray(da, U, &u));
         free(host);
     }
@@ -103,7 +108,8 @@ int main(int argc, char **argv)
         return 2;
     }
     PetscErrorCode e = PetscInitialize(&argc, &argv, NULL, NULL);
-    if (!e) e = run(atoi(argv[1]), atoi(argv[2]), atof(argv[3]), atof(argv[4]), argv[5], argv[6]);
+    if (!e) e = run(atoi(argv[1]), atoi(argv[2]), atof(argv[3]), atof(argv[4]), argv[5], argv[6],
+                    argc > 7 ? argv[7] : NULL);
     if (e) {
         fprintf(stderr, "error %d: %s\n", e, PetscLiteLastError());
         return 1;

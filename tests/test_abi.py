@@ -67,3 +67,21 @@ def test_invalid_arguments_are_reported(acx):
     g = acx.Grid(n=(16, 16, 16), h=0.1).c_struct()
     rc = lib.acx_stage_f64(C.byref(g), C.c_void_p(16), None, C.c_void_p(16), 0.0, 1.0, 1.0, None)
     assert rc == 1
+
+
+def test_compensated_step_rejects_aliased_arrays(acx):
+    """The six arrays of the FP32 compensated step (and of the selective
+    step) must be pairwise distinct: the 2.5D march reads HI and the
+    increments of neighbouring tiles as halo while other tiles write."""
+    lib = acx.lib()
+    g = acx.Grid(n=(16, 16, 16), h=0.1).c_struct()
+    base = [C.c_void_p(0x1000 * (k + 1)) for k in range(6)]
+    flags = C.c_void_p(0x9000)
+    for i in range(6):
+        for j in range(i):
+            arr = list(base)
+            arr[i] = arr[j]
+            rc = lib.acx_ssp_rks3_step_f32c(C.byref(g), *arr, 1e-3, None)
+            assert rc == 1 and b"must be distinct" in lib.acx_last_error(), (i, j)
+            rc = lib.acx_ssp_rks3_step_f32c_sel(C.byref(g), *arr, flags, 1e-3, None)
+            assert rc == 1 and b"must be distinct" in lib.acx_last_error(), (i, j)

@@ -9,8 +9,10 @@ on the GPU as one-plane grids, n = (Nx, Ny, 1) with a periodic z axis.
 * The reference 2D program (2d/weno4/standing_wave/hype.c, TSSSP RK104,
   adiabatic walls, T = 1) through petsclite reproduces
   results/Error.dat (tests/golden/standing_wave_2d_error.json).
-* Nonlinear weights are refused on 2D grids (the 3D nonlinear scheme on one
-  plane is not the 2D CWENO4 of 2d/weno4/bubble).
+* Nonlinear weights: the 2D CWENO4 of 2d/weno4/bubble/reconstruction.c:128-251
+  (five quadratic stencils) against the oracle's 2D CWENO4 (orc_cweno2d,
+  pinned by the stencil-footprint and jump-selection checks), RHS and steps
+  at 1e-12, FP32 compensated against FP64.
 """
 import ctypes as C
 import json
@@ -30,8 +32,9 @@ def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
 
-def oracle_rhs2d(orc, N, h, system, bc, U2):
-    g2 = orc.G2((N[0], N[1]), h, system, 0, 1.0, 0.0, 1.0, 1.0, bc)
+def oracle_rhs2d(orc, N, h, system, bc, U2, gamma=(1.0, 0.0)):
+    nonlinear = 0 if gamma == (1.0, 0.0) else 1
+    g2 = orc.G2((N[0], N[1]), h, system, nonlinear, gamma[0], gamma[1], 1.0, 1.0, bc)
     R = np.zeros_like(U2)
     assert orc.lib().orc_rhs2d_faithful(C.byref(g2), orc.P(U2), orc.P(R)) == 0
     return R, g2
@@ -89,12 +92,77 @@ def test_fp32_compensated_on_2d_grid(acx, torch_gpu):
     assert rel < 1e-6
 
 
-def test_nonlinear_weights_refused_on_2d_grid(acx, torch_gpu):
+def _field2d(seed, N, nv2, kind):
+    rng = np.random.default_rng(seed)
+    if kind == "rough":
+        return rng.standard_normal((N[1], N[0], nv2))
+    # a disc with a jump plus noise: the weights leave the linear weights along the rim
+    y, x = np.meshgrid(np.arange(N[1]), np.arange(N[0]), indexing="ij")
+    inside = (x - N[0] / 2) ** 2 + 1.4 * (y - N[1] / 2) ** 2 < (min(N) / 3) ** 2
+    U = np.zeros((N[1], N[0], nv2))
+    for v in range(nv2):
+        U[..., v] = np.where(inside, 1.0 + 0.3 * v, 0.2) + 0.01 * rng.standard_normal((N[1], N[0]))
+    return U
+
+
+@pytest.mark.parametrize("system,bc,gamma,kind", [
+    (0, (0, 0), (0.85, 0.85), "rough"), (0, (1, 1), (0.85, 0.85), "disc"), (0, (1, 0), (0.85, 0.85), "rough"),
+    (1, (0, 0), (0.85, 0.85), "disc"), (1, (0, 0), (0.85, 0.85), "rough"), (1, (0, 0), (0.5, 0.3), "rough"),
+    (0, (0, 0), (0.9, 0.1), "disc")],
+    ids=["wave2-per", "wave2-adia-disc", "wave2-mixed", "acoustic-disc", "acoustic-rough", "acoustic-g05-03",
+         "wave2-g09-01"])
+def test_2d_cweno4_matches_2d_oracle(system, bc, gamma, kind, acx, orc, torch_gpu):
+    """The 2D CWENO4 (2d/weno4/bubble/reconstruction.c:128-251) on one-plane
+    grids: RHS and three SSP-RK3 steps against the oracle's 2D RHS with
+    nonlinear weights.  gamma = (0.5, 0.3) makes the central linear weight
+    of the 3D form negative (its share of the two z stencils exceeds it):
+    only the sum enters, and the result still matches."""
     torch = torch_gpu
-    grid = acx.Grid(n=(16, 16, 1), h=1.0 / 16, system=acx.ACOUSTIC4, gamma_hi=0.85, gamma_lo=0.85)
-    U = torch.zeros(grid.shape(), dtype=torch.float64, device="cuda")
-    with pytest.raises(acx.AcxError, match="2D grid"):
-        acx.rhs(grid, U)
+    N, h = (36, 28), 1.0 / 36
+    nv2 = 2 if system == 0 else 3
+    U2 = _field2d(21 + system, N, nv2, kind)
+    U = np.zeros((1, N[1], N[0], 2 if system == 0 else 4))
+    U[0, ..., :nv2] = U2
+    grid = acx.Grid(n=(N[0], N[1], 1), h=h, system=system, bc=(bc[0], bc[1], 0), gamma_hi=gamma[0],
+                    gamma_lo=gamma[1])
+    print(f"[case] 2d cweno4 system={system} bc={bc} gamma={gamma} {kind}", flush=True)
+    R2, g2 = oracle_rhs2d(orc, N, h, system, bc, U2, gamma)
+    got = acx.rhs(grid, torch.from_numpy(U).cuda()).cpu().numpy()
+    err = rel_l2(got[0, ..., :nv2], R2)
+    lin, _ = oracle_rhs2d(orc, N, h, system, bc, U2)
+    print(f"[case] rhs rel_l2 {err:.3e} (linear scheme differs by {rel_l2(lin, R2):.2e})", flush=True)
+    assert err <= TOL
+    assert rel_l2(lin, R2) > 1e-6        # the weights did leave the linear weights
+    dt = acx.cfl_dt(grid, 0.4)
+    ref = U2.copy()
+    for _ in range(3):
+        orc.ts_solve("orc_rhs2d_cb", g2, ref, dt, dt, 0, 4)
+    Ud = torch.from_numpy(U).cuda()
+    for _ in range(3):
+        acx.ssp_rks3_step(grid, Ud, dt)
+    got = Ud.cpu().numpy()
+    err = rel_l2(got[0, ..., :nv2], ref)
+    print(f"[case] steps rel_l2 {err:.3e}", flush=True)
+    assert err <= TOL
+    if system == 1:
+        assert np.all(got[..., 3] == 0.0)
+
+
+def test_2d_cweno4_fp32_compensated(acx, torch_gpu):
+    torch = torch_gpu
+    N = 64
+    grid = acx.Grid(n=(N, N, 1), h=1.0 / N, system=acx.ACOUSTIC4, gamma_hi=0.85, gamma_lo=0.85)
+    U = np.zeros(grid.shape())
+    U[0, ..., :3] = _field2d(5, (N, N), 3, "disc") * 1e-3
+    dt = acx.cfl_dt(grid, 0.4)
+    ref = torch.from_numpy(U).cuda()
+    st = acx.CompensatedState(grid, torch.from_numpy(U).cuda())
+    for _ in range(20):
+        acx.ssp_rks3_step(grid, ref, dt)
+        st.step(dt)
+    rel, mx = acx.diff_f32c(st.hi, st.lo, ref)
+    print(f"[case] 2d cweno4 f32c rel_l2={rel:.2e} max={mx:.2e}", flush=True)
+    assert rel < 1e-6
 
 
 @pytest.mark.parametrize("N", [16, 32, 64])

@@ -148,7 +148,7 @@ def test_fused_indicator_equals_features_plus_forward(acx, torch_gpu):
     packed = learn.pack(p)
     X = learn.features(grid, U)
     Y = learn.forward(packed, X).cpu().numpy().reshape(-1, 4)
-    st = learn.SelectiveStepper(grid, packed, U.to(torch.float64), thr=0.5, dilate=0)
+    st = learn.SelectiveStepper(grid, packed, U.to(torch.float64), thr=0.5, dilate=0, judge=False)
     st.refresh_flags()
     fl = st.flags.cpu().numpy()
     want = ((Y > 0.5).astype(np.uint8) * np.array([1, 2, 4, 8], np.uint8)).sum(1)
@@ -307,3 +307,56 @@ def test_tile_mask_and_masked_step(acx, torch_gpu):
         a.step(dt, flags=flags)
         b.step(dt, flags=flags, mask=mt)
     assert torch.equal(a.hi, b.hi) and torch.equal(a.lo, b.lo)
+
+
+def test_judge_catches_a_front_that_leaves_its_flags(acx, torch_gpu):
+    """Full WENO judges every window of selective steps.  With a network that
+    flags nothing (threshold 1) and no dilation, the flags of a window are
+    only the cells the weights flag at its start; the steep front moves out
+    of them within the 12 steps, the judge finds cells the window ran with
+    the linear polynomial whose CWENO4 face averages differ from it, and
+    reruns the window with full CWENO4.  The main run
+    therefore equals the all-WENO FP32 stepper bit for bit, while the same
+    run without the judge does not."""
+    torch = torch_gpu
+    from acoustic_petsc_amd import learn
+    N = 48
+    g = acx.Grid(n=(N, N, N), h=1.0 / N, gamma_hi=0.85, gamma_lo=0.85)
+    U0 = torch.from_numpy(learn.steep_pulse_field(g, 77)).cuda()
+    dt = acx.cfl_dt(g, 0.4)
+    packed = learn.pack(torch.from_numpy(learn.init_params(0)).cuda())
+    judged = learn.SelectiveStepper(g, packed, U0, thr=1.0, refresh=12, dilate=0)
+    blind = learn.SelectiveStepper(g, packed, U0, thr=1.0, refresh=12, dilate=0, judge=False)
+    full = acx.CompensatedState(g, U0)
+    for _ in range(24):
+        judged.step(dt)
+        blind.step(dt)
+        full.step(dt)
+    misses = judged.finish()
+    print(f"[case] judge: {misses} misses (largest skipped face difference {judged.worst_skipped:.2e}, "
+          f"tolerance {judged.judge_tol:.2e}), {judged.redone_windows} of {judged.judged_windows} windows rerun",
+          flush=True)
+    assert misses > 0 and judged.redone_windows == 2
+    assert torch.equal(judged.state.hi, full.hi) and torch.equal(judged.state.lo, full.lo)
+    e_blind = acx.diff_f32c(blind.state.hi, blind.state.lo, full.value64())[0]
+    print(f"[case] without the judge: rel L2 {e_blind:.2e} against all-WENO", flush=True)
+    assert e_blind > 1e-7
+
+
+def test_judge_labels_match_the_fp64_labels(acx, torch_gpu):
+    """The judge's FP32 labels agree with the FP64 label kernel (the training
+    labels) except where the weight deviation sits within FP32 rounding of
+    the threshold."""
+    torch = torch_gpu
+    from acoustic_petsc_amd import learn
+    N = 40
+    g = acx.Grid(n=(N, N, N), h=1.0 / N, gamma_hi=0.85, gamma_lo=0.85)
+    U = torch.from_numpy(learn.steep_pulse_field(g, 8)).cuda()
+    lab64, dev = learn.labels(g, U, 0.05)
+    out, _, _ = learn.judge(g, U.to(torch.float32))
+    bits = torch.stack([(out >> v) & 1 for v in range(4)], dim=1).reshape(-1)
+    differ = bits != lab64
+    near = (dev - 0.05).abs() < 1e-4
+    print(f"[case] judge vs FP64 labels: {int(differ.sum())} differ, {int(lab64.sum())} labelled", flush=True)
+    assert int(lab64.sum()) > 0
+    assert bool((~differ | near).all())
