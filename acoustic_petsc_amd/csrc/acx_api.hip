@@ -8,6 +8,7 @@
 #include <math.h>
 #include "acx_device.h"
 #include "acx_internal.h"
+#include "../../include/acx_learn.h"
 
 namespace acx {
 
@@ -598,5 +599,59 @@ extern "C" int acx_ssp_rks3_step_f32c_selm(const acx_grid *g, const float *HI, c
     if (!mask) return set_error(ACX_EINVAL, "selective step: mask is NULL");
     hipStream_t s = (hipStream_t)stream;
     if (env_int("ACX_SEL_FULL", 0)) return sel_stages(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags, nullptr, dt, s);
+    return sel_stages(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags, mask, dt, s);
+}
+
+// ---- selective step without allocation -------------------------------------
+//
+// The tile mask lives in a file-scope device array of fixed size; each device
+// holds its own copy.  Host code never passes the array's name to a kernel
+// (on the host it is only a stand-in symbol): the device address is looked up
+// once per device with hipGetSymbolAddress and kept in g_scratch_ptr.
+static __device__ __align__(256) uint8_t scratch[ACX_SEL_NOALLOC_MAX_TILES];
+static uint8_t *g_scratch_ptr[64];
+static std::mutex g_scratch_mu;
+
+static int scratch_for_current(uint8_t **out)
+{
+    int dev = 0;
+    int rc = hip_check(hipGetDevice(&dev), "selective step (no allocation): device");
+    if (rc) return rc;
+    if (dev < 0 || dev >= 64) return set_error(ACX_EUNSUPPORTED, "selective step (no allocation): device %d", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if ((rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "selective step (no allocation): scratch address")))
+            return rc;
+        g_scratch_ptr[dev] = (uint8_t *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return ACX_OK;
+}
+
+extern "C" int acx_ssp_rks3_step_f32c_sel_noalloc(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
+                                                  float *LO_OUT, float *D0, float *D1, const uint8_t *flags,
+                                                  double dt, void *stream)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!flags) return set_error(ACX_EINVAL, "flags is NULL");
+    if ((rc = check_f32c_arrays(HI, LO, HI_OUT, LO_OUT, D0, D1))) return rc;
+    if (g->system != ACX_ACOUSTIC4) return set_error(ACX_EUNSUPPORTED, "selective step: ACOUSTIC4 only");
+    CwArgs ca;
+    fill_cw_patch(ca, g);
+    const size_t tiles = cweno_sel_tiles(ca);
+    if (tiles > ACX_SEL_NOALLOC_MAX_TILES)
+        return set_error(ACX_EINVAL, "selective step (no allocation): %zu patch tiles, at most %d fit the static scratch",
+                         tiles, ACX_SEL_NOALLOC_MAX_TILES);
+    // the watch word is not created here (that would allocate): it is used
+    // when another entry point has created it on this device
+    if (nonfinite_raised())
+        return set_error(ACX_ENONFINITE, "selective step: a kernel on this device wrote a NaN or Inf");
+    uint8_t *mask = nullptr;
+    if ((rc = scratch_for_current(&mask))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    // every mask entry of the grid's tiles is written by the marking kernel
+    if ((rc = launch_cweno_tilemask(ca, flags, mask, s))) return rc;
     return sel_stages(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags, mask, dt, s);
 }

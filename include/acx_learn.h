@@ -105,6 +105,21 @@ int acx_flags_count(const uint8_t *flags, size_t ncells, long long *bits, void *
 int acx_ssp_rks3_step_f32c_sel(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
                                float *LO_OUT, float *D0, float *D1, const uint8_t *flags, double dt, void *stream);
 
+/* acx_ssp_rks3_step_f32c_sel without any allocation.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The tile
+ * mask is kept in a fixed static device array of ACX_SEL_NOALLOC_MAX_TILES
+ * bytes per GPU (held until the process ends): grids with more patch tiles
+ * (acx_sel_mask_bytes) are rejected with ACX_EINVAL.  Calls on one GPU share
+ * that array, so calls that overlap on one GPU (other streams or threads)
+ * race; the mask is rewritten on every call.  The non-finite watch applies
+ * once another entry point has created the device's watch word.  Results
+ * equal acx_ssp_rks3_step_f32c_sel bit for bit. */
+#define ACX_SEL_NOALLOC_MAX_TILES 262144
+int acx_ssp_rks3_step_f32c_sel_noalloc(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
+                                       float *LO_OUT, float *D0, float *D1, const uint8_t *flags, double dt,
+                                       void *stream);
+
 /* The tile mask of the selective step for these flags: acx_sel_mask_bytes(g)
  * bytes (0 for an invalid grid), one per patch tile, 1 where the tile holds a
  * flagged cell or a face neighbour of one. */
