@@ -342,13 +342,15 @@ int acx_ssp_rks3_step_f64(const acx_grid *g, double *U, double *W0, double *W1, 
 
 namespace acx {
 
-// The four stage launches of one FP64 step, captured once per (device,
-// grid, arrays, dt) and replayed; one cached graph per device.
+// The stage launches of `steps` FP64 steps, captured once per (device,
+// grid, arrays, dt, steps) and replayed; one cached graph per device.
+constexpr int kMaxGraphSteps = 32;
 struct StepGraph {
     bool valid = false;
     acx_grid g;
     double *U = nullptr, *W0 = nullptr, *W1 = nullptr;
     double dt = 0.0;
+    int steps = 0;
     hipGraphExec_t exec = nullptr;
 };
 static StepGraph g_step_graph[64];
@@ -356,14 +358,16 @@ static std::mutex g_step_graph_mu;
 
 static bool same_grid(const acx_grid &a, const acx_grid &b) { return memcmp(&a, &b, sizeof(acx_grid)) == 0; }
 
-static int step_graph_for(const acx_grid *g, double *U, double *W0, double *W1, double dt, hipGraphExec_t *out)
+static int step_graph_for(const acx_grid *g, double *U, double *W0, double *W1, double dt, int steps,
+                          hipGraphExec_t *out)
 {
     int dev = 0;
     int rc = hip_check(hipGetDevice(&dev), "step graph: device");
     if (rc) return rc;
     if (dev < 0 || dev >= 64) return set_error(ACX_EUNSUPPORTED, "step graph: device %d", dev);
     StepGraph &sg = g_step_graph[dev];
-    if (sg.valid && same_grid(sg.g, *g) && sg.U == U && sg.W0 == W0 && sg.W1 == W1 && sg.dt == dt) {
+    if (sg.valid && same_grid(sg.g, *g) && sg.U == U && sg.W0 == W0 && sg.W1 == W1 && sg.dt == dt &&
+        sg.steps == steps) {
         *out = sg.exec;
         return ACX_OK;
     }
@@ -377,12 +381,15 @@ static int step_graph_for(const acx_grid *g, double *U, double *W0, double *W1, 
     hipGraph_t graph = nullptr;
     rc = hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed), "step graph: begin capture");
     if (!rc) {
-        // the stage sequence of acx_ssp_rks3_step_f64
-        rc = stage64(g, U, nullptr, W0, 0.0, 1.0, dt / 2, EPI_AXPY, cs, false);
-        if (!rc) rc = stage64(g, W0, nullptr, W1, 0.0, 1.0, dt / 2, EPI_AXPY, cs, false);
-        if (!rc)
-            rc = stage64(g, W1, U, W0, 1. * 2 / 3, (2 - 1.) * 1. / 3, (2 - 1.) * dt / (2 * 3), EPI_COMB, cs, false);
-        if (!rc) rc = stage64(g, W0, nullptr, U, 0.0, 1.0, dt / 2, EPI_AXPY, cs);
+        // the stage sequence of acx_ssp_rks3_step_f64, `steps` times
+        for (int k = 0; k < steps && !rc; ++k) {
+            rc = stage64(g, U, nullptr, W0, 0.0, 1.0, dt / 2, EPI_AXPY, cs, false);
+            if (!rc) rc = stage64(g, W0, nullptr, W1, 0.0, 1.0, dt / 2, EPI_AXPY, cs, false);
+            if (!rc)
+                rc = stage64(g, W1, U, W0, 1. * 2 / 3, (2 - 1.) * 1. / 3, (2 - 1.) * dt / (2 * 3), EPI_COMB, cs,
+                             false);
+            if (!rc) rc = stage64(g, W0, nullptr, U, 0.0, 1.0, dt / 2, EPI_AXPY, cs);
+        }
         const int re = hip_check(hipStreamEndCapture(cs, &graph), "step graph: end capture");
         if (!rc) rc = re;
     }
@@ -395,6 +402,7 @@ static int step_graph_for(const acx_grid *g, double *U, double *W0, double *W1, 
     sg.W0 = W0;
     sg.W1 = W1;
     sg.dt = dt;
+    sg.steps = steps;
     sg.valid = true;
     *out = sg.exec;
     return ACX_OK;
@@ -412,12 +420,21 @@ int acx_ssp_rks3_steps_graph_f64(const acx_grid *g, double *U, double *W0, doubl
     if (!U || !W0 || !W1 || U == W0 || U == W1 || W0 == W1) return set_error(ACX_EINVAL, "U, W0, W1 must be distinct");
     if (nsteps < 0) return set_error(ACX_EINVAL, "nsteps must be >= 0");
     if ((rc = nonfinite_entry("ssp_rks3_steps_graph_f64"))) return rc;   // creates the watch word before capture
+    if (nsteps == 0) return ACX_OK;
     std::lock_guard<std::mutex> lk(g_step_graph_mu);
+    // one graph of up to kMaxGraphSteps steps, replayed; the remainder from
+    // a graph of its own size (both are cached in turn)
+    const int per = nsteps < kMaxGraphSteps ? nsteps : kMaxGraphSteps;
     hipGraphExec_t exec = nullptr;
-    if ((rc = step_graph_for(g, U, W0, W1, dt, &exec))) return rc;
-    for (int k = 0; k < nsteps; ++k)
+    if ((rc = step_graph_for(g, U, W0, W1, dt, per, &exec))) return rc;
+    for (int k = 0; k + per <= nsteps; k += per)
         if ((rc = hip_check(hipGraphLaunch(exec, (hipStream_t)stream), "step graph: launch"))) return rc;
-    return ACX_OK;
+    const int rem = nsteps % per;
+    if (rem) {
+        if ((rc = step_graph_for(g, U, W0, W1, dt, rem, &exec))) return rc;
+        rc = hip_check(hipGraphLaunch(exec, (hipStream_t)stream), "step graph: launch");
+    }
+    return rc;
 }
 
 int acx_ssp_rks3_step_f32c(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,

@@ -79,12 +79,12 @@ int acx_stage_f64(const acx_grid *g, const double *Q, const double *U0, double *
  * W0, W1 are device work vectors of the same size as U. */
 int acx_ssp_rks3_step_f64(const acx_grid *g, double *U, double *W0, double *W1, double dt, void *stream);
 
-/* nsteps steps of acx_ssp_rks3_step_f64 replayed from a hipGraph of the four
+/* nsteps steps of acx_ssp_rks3_step_f64 replayed from a hipGraph of their
  * stage launches (the reference's stage loop, hype.c:180-183, PETSc's
- * TSStep_SSP), captured on the first call for a (device, grid, U, W0, W1,
- * dt) and reused while they stay the same (one cached graph per device).
- * For small grids, where the launch overhead is a large part of a stage.
- * Results equal acx_ssp_rks3_step_f64 bit for bit. */
+ * TSStep_SSP): a graph of min(nsteps, 32) steps is captured for a (device,
+ * grid, U, W0, W1, dt) and replayed, the remainder from a graph of its own
+ * size (one cached graph per device, captured again when any of these
+ * change).  Results equal acx_ssp_rks3_step_f64 bit for bit. */
 int acx_ssp_rks3_steps_graph_f64(const acx_grid *g, double *U, double *W0, double *W1, double dt, int nsteps,
                                  void *stream);
 
