@@ -569,4 +569,204 @@ __device__ __forceinline__ void linear_tile_centre(const LinArgs &p, int x0, int
     watch_flush(p.nf, nfbad);
 }
 
+// ---------------------------------------------------------------------------
+// LDS-DMA march (FP64 ACOUSTIC4, all-periodic grids, RHS and AXPY stages).
+//
+// The same tile (32 x 16 columns) and arithmetic as the centre-plane march,
+// with the plane traffic moved out of registers: every plane of the tile
+// with its 3-cell x/y halo (22 x 38 cells, corners included) is copied from
+// HBM straight into an LDS ring slot by direct-to-LDS loads (16 bytes per
+// lane, global_load_lds_dwordx4), six slots deep (160.5 KB).  At iteration z
+// the slots hold planes z .. z+5: plane z is the in-plane stencil's, planes
+// z+1 .. z+3 have landed, z+4 and z+5 are in flight.  The z-stencil keeps
+// the own column's planes z-3 .. z+3 in registers, the newest read from the
+// slot of plane z+3 once it has landed.  So each plane's loads are issued
+// five iterations before its stencil and two before its column value is
+// needed, and two planes per CU are in flight while a plane is computed,
+// without holding them in registers.  (Measured on MI355X at 256^3: loading
+// each slot again right after a second barrier in the middle of the
+// iteration, one plane deeper, and issuing the stores before the loads were
+// both slower.)
+//
+// The loads are issued from inline assembly so that the compiler does not
+// guard every LDS read with a wait for all of them; the waits are explicit:
+// loads complete in issue order, and at iteration z the only loads issued
+// after plane z+3's are those of plane z+4 (nj per wavefront), so vmcnt <= nj
+// means plane z+3 has landed in this wavefront's part (stores count too,
+// which can only lengthen the wait); the barrier that follows makes every
+// wavefront's part visible and ends all reads of the slot the next loads
+// overwrite.  Plane layout in a slot: two
+// arrays of 22 x 38 16-byte entries, the (p, u) halves then the (v, w)
+// halves of the cells, row-major; one wavefront load fills 64 consecutive
+// entries.
+struct DmaPlan {
+    static constexpr int BX = 32, BY = 16, RW = BX + 6, RH = BY + 6;
+    static constexpr int NE = RW * RH;              // 836 cells per plane
+    static constexpr int NENT = 2 * NE;             // 1672 sixteen-byte entries
+    static constexpr int SLOT = NENT * 16;          // 26752 bytes
+    static constexpr int NSLOT = 6;
+    static constexpr int NCHUNK = (NENT + 63) / 64; // 27 wavefront loads per plane
+    static constexpr int NWAVE = BX * BY / 64;      // 8
+    static constexpr int MAXJ = (NCHUNK + NWAVE - 1) / NWAVE;   // 4 loads per wavefront at most
+};
+
+template <int EPI>
+__device__ __forceinline__ void linear_tile_dma(const LinArgs &p, int x0, int y0, int zb, unsigned char *ring)
+{
+    using D = DmaPlan;
+    static_assert(EPI == EPI_RHS || EPI == EPI_AXPY, "LDS-DMA march: RHS and AXPY stages");
+    const double *__restrict__ Q = static_cast<const double *>(p.q);
+    double *__restrict__ OUT = static_cast<double *>(p.out);
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * D::BX + tx;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int ze = min(zb + p.zc, nz);
+    const bool active = (x0 + tx < nx) && (y0 + ty < ny);
+    const size_t plane = (size_t)nx * ny;
+
+    // this lane's entries of a plane: chunk q = wave + 8 j covers entries 64 q .. 64 q + 63
+    const int nj = wave + D::NWAVE * (D::MAXJ - 1) < D::NCHUNK ? D::MAXJ : D::MAXJ - 1;
+    uint32_t eoff[D::MAXJ];
+    bool eok[D::MAXJ];
+#pragma unroll
+    for (int j = 0; j < D::MAXJ; ++j) {
+        const int e = (wave + D::NWAVE * j) * 64 + lane;
+        eok[j] = j < nj && e < D::NENT;
+        const int h = e / D::NE, idx = e - h * D::NE, r = idx / D::RW, c = idx - r * D::RW;
+        int gx = x0 + c - 3, gy = y0 + r - 3;
+        gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+        gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+        eoff[j] = eok[j] ? (uint32_t)(((size_t)gy * nx + gx) * 4 + 2 * h) : 0u;
+    }
+    auto slot_of = [&](int z) { return (z - zb + 3) % D::NSLOT; };
+    auto issue = [&](int z, int slot) {
+        int zz = z % nz;
+        zz = zz < 0 ? zz + nz : zz;
+        const double *src = Q + (size_t)zz * plane * 4;
+        unsigned char *dst = ring + slot * D::SLOT;
+#pragma unroll
+        for (int j = 0; j < D::MAXJ; ++j) {
+            if (j < nj) {
+                // LDS byte offset of the chunk (wave-uniform) into M0; the
+                // lane's 16 bytes land at M0 + 16 * lane
+                const unsigned lds = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(
+                    (__attribute__((address_space(3))) unsigned char *)(dst + (wave + D::NWAVE * j) * 1024)));
+                const double *g = src + eoff[j];
+                if (eok[j])
+                    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                                 :: "s"(lds), "v"(g) : "memory", "m0");
+            }
+        }
+    };
+    // s_waitcnt with only vmcnt constrained (gfx9 encoding: expcnt 7,
+    // lgkmcnt 15): all but the loads of the `later` most recent planes
+    auto wait_planes = [&](int later) {
+        if (later == 0) {
+            __builtin_amdgcn_s_waitcnt(0x0F70);
+        } else if (later == 1) {
+            if (nj == 4) __builtin_amdgcn_s_waitcnt(0x0F74);
+            else __builtin_amdgcn_s_waitcnt(0x0F73);
+        } else {
+            if (nj == 4) __builtin_amdgcn_s_waitcnt(0x0F78);
+            else __builtin_amdgcn_s_waitcnt(0x0F76);
+        }
+    };
+    auto cell_at = [&](int slot, int r, int c) -> Cell<double, 4> {
+        const unsigned char *b = ring + slot * D::SLOT;
+        const double2 a = *reinterpret_cast<const double2 *>(b + (r * D::RW + c) * 16);
+        const double2 v = *reinterpret_cast<const double2 *>(b + (D::NE + r * D::RW + c) * 16);
+        Cell<double, 4> o;
+        o.v[0] = a.x; o.v[1] = a.y; o.v[2] = v.x; o.v[3] = v.y;
+        return o;
+    };
+    auto half_at = [&](int slot, int h, int r, int c) -> double2 {
+        return *reinterpret_cast<const double2 *>(ring + slot * D::SLOT + (h * D::NE + r * D::RW + c) * 16);
+    };
+
+    // prologue: planes zb-3 .. zb+2 into slots 0 .. 5, their own-column values
+    // into the z-queue, then planes zb+3 and zb+4 into the slots of zb-3, zb-2
+    for (int d = 0; d < D::NSLOT; ++d) issue(zb - 3 + d, d);
+    wait_planes(0);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    Cell<double, 4> qz[7];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) qz[d + 1] = cell_at(d, ty + 3, tx + 3);
+    qz[0] = qz[1];
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    issue(zb + 3, slot_of(zb + 3));
+    if (zb + 4 <= ze + 2) issue(zb + 4, slot_of(zb + 4));
+
+    const double inv48h = p.inv48h, Kc = p.K, irho = p.invrho, sLLF = p.s;
+    const size_t col = (size_t)(y0 + ty) * nx + (x0 + tx);
+    bool nfbad = false;
+    for (int z = zb; z < ze; ++z) {
+        // plane z+3 landed: the loads issued after its loads are plane z+4's
+        // (when issued: planes beyond ze+2 are not needed)
+        wait_planes(z + 4 <= ze + 2);
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        // every wavefront is past iteration z-1: its slot takes plane z+5
+        if (z + 5 <= ze + 2) issue(z + 5, slot_of(z + 5));
+        const int s0 = slot_of(z);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) qz[k] = qz[k + 1];
+        qz[6] = cell_at(slot_of(z + 3), ty + 3, tx + 3);
+
+        // in-plane stencils from the slot of plane z, one 16-byte half (two
+        // components) at a time; z-stencils from the queue
+        double accB[4], dAp[3], dAvel = 0.0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            double2 xr[7], yc[7];
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                xr[k] = half_at(s0, h, ty + 3, tx + k);
+                yc[k] = half_at(s0, h, ty + k, tx + 3);
+            }
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                const int v = 2 * h + w;
+                auto X = [&](int k) { return w ? xr[k].y : xr[k].x; };
+                auto Y = [&](int k) { return w ? yc[k].y : yc[k].x; };
+                const double q0 = qz[3].v[v];
+                accB[v] = stencil_B7(X(0), X(1), X(2), q0, X(4), X(5), X(6)) +
+                          stencil_B7(Y(0), Y(1), Y(2), q0, Y(4), Y(5), Y(6)) +
+                          stencil_B7(qz[0].v[v], qz[1].v[v], qz[2].v[v], q0, qz[4].v[v], qz[5].v[v], qz[6].v[v]);
+                if (v == 0) {
+                    dAp[0] = stencil_A7(X(0), X(1), X(2), X(4), X(5), X(6));
+                    dAp[1] = stencil_A7(Y(0), Y(1), Y(2), Y(4), Y(5), Y(6));
+                    dAp[2] = stencil_A7(qz[0].v[0], qz[1].v[0], qz[2].v[0], qz[4].v[0], qz[5].v[0], qz[6].v[0]);
+                } else if (v == 1) {
+                    dAvel += stencil_A7(X(0), X(1), X(2), X(4), X(5), X(6));
+                } else if (v == 2) {
+                    dAvel += stencil_A7(Y(0), Y(1), Y(2), Y(4), Y(5), Y(6));
+                } else {
+                    dAvel += stencil_A7(qz[0].v[3], qz[1].v[3], qz[2].v[3], qz[4].v[3], qz[5].v[3], qz[6].v[3]);
+                }
+            }
+        }
+        Cell<double, 4> r;
+        r.v[0] = inv48h * (sLLF * accB[0] - Kc * dAvel);
+        r.v[1] = inv48h * (sLLF * accB[1] - irho * dAp[0]);
+        r.v[2] = inv48h * (sLLF * accB[2] - irho * dAp[1]);
+        r.v[3] = inv48h * (sLLF * accB[3] - irho * dAp[2]);
+        if (active) {
+            Cell<double, 4> o;
+            if constexpr (EPI == EPI_RHS) {
+                o = r;
+            } else {
+                const double cc = p.c;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) o.v[v] = qz[3].v[v] + cc * r.v[v];
+            }
+            if (p.nf) finite_acc<double, 4>(nfbad, o);
+            store_cell_nt<double, 4>(OUT, (size_t)z * plane + col, o);
+        }
+    }
+    // every load issued has been waited for by the last iteration (vmcnt 0)
+    watch_flush(p.nf, nfbad);
+}
+
 } // namespace acx
