@@ -39,15 +39,22 @@ k_ensemble(const EnsProblem *__restrict__ probs, int nprob, int total, unsigned 
     using Tile = LinTile<float, ACX_ACOUSTIC4, ENS_BX, BY>;
     __shared__ float lds[2][Tile::NS][BY + 6][ENS_BX + 6];
     __shared__ int s_item;
+    __shared__ int s_first[ENS_MAXP];   // first work item of every problem (ascending)
     const int tid = threadIdx.y * ENS_BX + threadIdx.x;
+    for (int i = tid; i < nprob; i += ENS_BX * BY) s_first[i] = probs[i].first;
     for (;;) {
         if (tid == 0) s_item = (int)atomicAdd(counter, 1u);
         __syncthreads();
         const int item = s_item;
         __syncthreads();   // every thread has read s_item before it is overwritten
         if (item >= total) break;
-        int p = 0;
-        while (p + 1 < nprob && probs[p + 1].first <= item) ++p;
+        // the problem of the item: the last p with s_first[p] <= item (binary search)
+        int p = 0, hi = nprob - 1;
+        while (p < hi) {
+            const int mid = (p + hi + 1) >> 1;
+            if (s_first[mid] <= item) p = mid;
+            else hi = mid - 1;
+        }
         const EnsProblem pr = probs[p];
         const int local = item - pr.first;
         LinArgs a = base;

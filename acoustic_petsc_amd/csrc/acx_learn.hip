@@ -145,55 +145,89 @@ __global__ void __launch_bounds__(256) k_labels(const double *__restrict__ U, Cw
 // max of non-negative values).  Independently, label bit v of a cell is set
 // when the eight normalised weights of component v leave the linear weights
 // by more than `thr` (the training label), and out = base | labels.
-__global__ void __launch_bounds__(256) k_judge(const float *__restrict__ HI, CwArgs p, float thr, float tol,
-                                               const uint8_t *__restrict__ used, const uint8_t *__restrict__ base,
-                                               uint8_t *__restrict__ out, unsigned long long *__restrict__ missed,
-                                               unsigned *__restrict__ worst)
+// One block judges a JT_BX x JT_BY column tile over JT_ZC planes: the
+// stencil region of the current plane and of the two planes on either side
+// (2-cell x/y halo) sits in a 5-plane LDS ring, one array per component, so
+// the 33 values of every cell are read from LDS; plane z+3 is loaded while
+// plane z is judged.
+constexpr int JT_BX = 32, JT_BY = 8, JT_ZC = 16;
+constexpr int JT_LX = JT_BX + 4, JT_LY = JT_BY + 4;
+
+__global__ void __launch_bounds__(JT_BX *JT_BY) k_judge(const float *__restrict__ HI, CwArgs p, float thr, float tol,
+                                                          const uint8_t *__restrict__ used,
+                                                          const uint8_t *__restrict__ base, uint8_t *__restrict__ out,
+                                                          unsigned long long *__restrict__ missed,
+                                                          unsigned *__restrict__ worst)
 {
+    __shared__ float ring[5][4][JT_LY][JT_LX];
     const int nx = p.nx, ny = p.ny, nz = p.nz;
-    const size_t ncell = (size_t)nx * ny * nz;
-    const size_t cell = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const int ntx = (nx + JT_BX - 1) / JT_BX, nty = (ny + JT_BY - 1) / JT_BY;
+    const int b = blockIdx.x;
+    const int x0 = (b % ntx) * JT_BX, y0 = ((b / ntx) % nty) * JT_BY, zb = (b / (ntx * nty)) * JT_ZC;
+    const int ze = min(zb + JT_ZC, nz);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * JT_BX + tx;
+    const bool active = x0 + tx < nx && y0 + ty < ny;
+    const size_t plane = (size_t)nx * ny;
+    auto wrap = [](int q, int n) { return q < 0 ? q + n : (q >= n ? q - n : q); };
+    // the stencil region of plane zz into ring slot sl (float4 loads, one cell per thread and pass)
+    auto load_plane = [&](int zz, int sl) {
+        const size_t zo = (size_t)wrap(zz, nz) * plane;
+        for (int c = tid; c < JT_LX * JT_LY; c += JT_BX * JT_BY) {
+            const int ly = c / JT_LX, lx = c - ly * JT_LX;
+            const int gx = wrap(x0 + lx - 2, nx), gy = wrap(y0 + ly - 2, ny);
+            const float4 v = *reinterpret_cast<const float4 *>(HI + (zo + (size_t)gy * nx + gx) * 4);
+            ring[sl][0][ly][lx] = v.x;
+            ring[sl][1][ly][lx] = v.y;
+            ring[sl][2][ly][lx] = v.z;
+            ring[sl][3][ly][lx] = v.w;
+        }
+    };
+    // stencil slots of cweno_weights: centre, x/y/z arms (m2 m1 p1 p2), xy/yz/zx diagonals, corners
+    constexpr int DX[33] = {0, -2, -1, 1, 2, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, 1, -1, -1, 1, -1};
+    constexpr int DY[33] = {0, 0, 0, 0, 0, -2, -1, 1, 2, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 0, 0, 0, 0, 1, 1, -1, 1, -1, 1, -1, -1};
+    constexpr int DZ[33] = {0, 0, 0, 0, 0, 0, 0, 0, 0, -2, -1, 1, 2, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 1, 1, 1, -1, 1, -1, -1, -1};
+    for (int d = -2; d <= 2; ++d) load_plane(zb + d, d + 2);
     unsigned miss = 0;
     float wmax = 0.f;
-    if (cell < ncell) {
-        const int i = (int)(cell % nx), j = (int)((cell / nx) % ny), k = (int)(cell / ((size_t)nx * ny));
-        auto W = [](int q, int n) { return q < 0 ? q + n : (q >= n ? q - n : q); };
-        // stencil slots of cweno_weights: centre, x/y/z arms (m2 m1 p1 p2), xy/yz/zx diagonals, corners
-        const int dx[33] = {0, -2, -1, 1, 2, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, 1, -1, -1, 1, -1};
-        const int dy[33] = {0, 0, 0, 0, 0, -2, -1, 1, 2, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 0, 0, 0, 0, 1, 1, -1, 1, -1, 1, -1, -1};
-        const int dz[33] = {0, 0, 0, 0, 0, 0, 0, 0, 0, -2, -1, 1, 2, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 1, 1, 1, -1, 1, -1, -1, -1};
-        size_t off[33];
+    for (int z = zb; z < ze; ++z) {
+        __syncthreads();
+        const int s0 = (z - zb + 2) % 5;   // slot of plane z; plane z+d is in slot (s0 + d) mod 5
+        if (active) {
+            const size_t cell = (size_t)z * plane + (size_t)(y0 + ty) * nx + (x0 + tx);
+            const unsigned uflag = used ? used[cell] : 0xFu;
+            unsigned lab = 0;
+            for (int v = 0; v < 4; ++v) {
+                auto get = [&](int q) {
+                    return ring[(s0 + 5 + DZ[q]) % 5][v][ty + 2 + DY[q]][tx + 2 + DX[q]];
+                };
+                float wt, w3[7];
+                cweno_weights<float>(get, p, wt, w3);
+                float dev = fabsf(wt * float(p.g4) - float(p.g4)) + fabsf(w3[0] - float(p.g3c));
 #pragma unroll
-        for (int q = 0; q < 33; ++q)
-            off[q] = (((size_t)W(k + dz[q], nz) * ny + W(j + dy[q], ny)) * nx + W(i + dx[q], nx)) * 4;
-        const unsigned uflag = used ? used[cell] : 0xFu;
-        unsigned lab = 0;
-        for (int v = 0; v < 4; ++v) {
-            float st[33];
+                for (int q = 1; q < 7; ++q) dev += fabsf(w3[q] - float(p.g3s));
+                if (dev > thr) lab |= 1u << v;
+                if (!((uflag >> v) & 1u)) {
+                    // CWENO4 and linear face averages of the cell, largest difference over the six faces
+                    Axis3<float> o;
+                    cweno_combine<float>(get(0), get(1), get(2), get(3), get(4), get(5), get(6), get(7), get(8),
+                                         get(9), get(10), get(11), get(12), wt, w3, p, o);
+                    float d = 0.f;
 #pragma unroll
-            for (int q = 0; q < 33; ++q) st[q] = HI[off[q] + v];
-            Axis3<float> o;
-            float wn[8];
-            cweno_axes<float>(st, p, o, wn);
-            float dev = fabsf(wn[0] - float(p.g4)) + fabsf(wn[1] - float(p.g3c));
-#pragma unroll
-            for (int q = 2; q < 8; ++q) dev += fabsf(wn[q] - float(p.g3s));
-            if (dev > thr) lab |= 1u << v;
-            if (!((uflag >> v) & 1u)) {
-                float c1, c2, c3, d = 0.f;
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const int m2 = 1 + 4 * a;   // arm slots m2 m1 p1 p2 of axis a
-                    axis_linear(st[m2], st[m2 + 1], st[0], st[m2 + 2], st[m2 + 3], c1, c2, c3);
-                    const float dd = fabsf(o.c2[a] - c2) * (1.f / 6.f) +
-                                     fabsf(0.5f * (o.c1[a] - c1) + 0.05f * (o.c3[a] - c3));
-                    d = fmaxf(d, dd);
+                    for (int a = 0; a < 3; ++a) {
+                        float c1, c2, c3;
+                        const int m2 = 1 + 4 * a;   // arm slots m2 m1 p1 p2 of axis a
+                        axis_linear(get(m2), get(m2 + 1), get(0), get(m2 + 2), get(m2 + 3), c1, c2, c3);
+                        d = fmaxf(d, fabsf(o.c2[a] - c2) * (1.f / 6.f) +
+                                         fabsf(0.5f * (o.c1[a] - c1) + 0.05f * (o.c3[a] - c3)));
+                    }
+                    wmax = fmaxf(wmax, d);
+                    if (d > tol) ++miss;
                 }
-                wmax = fmaxf(wmax, d);
-                if (d > tol) ++miss;
             }
+            if (out) out[cell] = (uint8_t)((base ? base[cell] : 0) | lab);
         }
-        if (out) out[cell] = (uint8_t)((base ? base[cell] : 0) | lab);
+        __syncthreads();   // every thread is done with the slot of plane z-2
+        if (z + 1 < ze) load_plane(z + 3, (s0 + 3) % 5);
     }
     if (missed) {
 #pragma unroll
@@ -201,7 +235,7 @@ __global__ void __launch_bounds__(256) k_judge(const float *__restrict__ HI, CwA
             miss += __shfl_down(miss, o, 64);
             wmax = fmaxf(wmax, __shfl_down(wmax, o, 64));
         }
-        if ((threadIdx.x & 63) == 0) {
+        if ((tid & 63) == 0) {
             if (miss) atomicAdd(missed, (unsigned long long)miss);
             if (wmax > 0.f) atomicMax(worst, __float_as_uint(wmax));   // non-negative floats order as their bits
         }
@@ -617,15 +651,16 @@ int acx_weno_judge_f32(const acx_grid *g, const float *HI, const uint8_t *used, 
     p.g3c = (1.0 - g->gamma_hi) * g->gamma_lo;
     p.g3s = (1. / 6.) * (1 - g->gamma_hi) * (1 - g->gamma_lo);
     p.inv_g4 = 1.0 / g->gamma_hi;
-    const size_t nc = (size_t)g->n[0] * g->n[1] * g->n[2];
     hipStream_t s = (hipStream_t)stream;
     unsigned long long *d = nullptr;   // [0] misses, [1] low word: worst difference (float bits)
     if (missed) {
         if ((rc = hip_check(hipMallocAsync((void **)&d, 2 * sizeof(unsigned long long), s), "judge counter"))) return rc;
         hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, (float *)d, 4, 0.f);
     }
-    hipLaunchKernelGGL(k_judge, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, HI, p, (float)thr, (float)tol,
-                       used, base, out, d, d ? (unsigned *)(d + 1) : nullptr);
+    const unsigned blocks = (unsigned)(((g->n[0] + JT_BX - 1) / JT_BX) * ((g->n[1] + JT_BY - 1) / JT_BY) *
+                                       ((g->n[2] + JT_ZC - 1) / JT_ZC));
+    hipLaunchKernelGGL(k_judge, dim3(blocks), dim3(JT_BX, JT_BY), 0, s, HI, p, (float)thr, (float)tol, used, base, out,
+                       d, d ? (unsigned *)(d + 1) : nullptr);
     rc = hip_check(hipGetLastError(), "judge");
     if (missed) {
         unsigned long long h[2] = {0, 0};
