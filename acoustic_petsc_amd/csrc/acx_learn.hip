@@ -25,6 +25,7 @@ struct MlpPacked {
     bf16x8 w1[2][64];   // layer 1 A operand per 16-row half and lane
     bf16x8 w2[2][64];   // layer 2 A operand, K permuted to the layer-1 output order
     float b1[32], b2[32], w3[32], b3;
+    float act;          // hidden activation: 0 ReLU, 1 tanh
 };
 
 // output feature held in slot j (0..7) of a lane of group g
@@ -32,12 +33,15 @@ __host__ __device__ inline int kperm(int g, int j) { return j < 4 ? 4 * g + j : 
 
 __device__ __forceinline__ float bf16_round(float x) { return (float)(__bf16)x; }
 
-// Hidden activation: ReLU.  One VALU max per unit; an exponential-based
-// activation (tanh) made the fused inference pass bound by its transcendental
-// issue rate (two quarter-rate instructions per unit, 64 units per sample).
-// max(x, 0) as one integer max on the bit pattern: negative floats (and -0)
-// have negative int patterns; fmaxf would add a NaN-quieting step.
+// Hidden activations.  ReLU (the default): one VALU max per unit, max(x, 0)
+// as one integer max on the bit pattern (negative floats and -0 have
+// negative int patterns; fmaxf would add a NaN-quieting step).  tanh:
+// 1 - 2 / (exp(2x) + 1), an exponential and a reciprocal per unit, which
+// makes the fused inference pass bound by the quarter-rate transcendental
+// issue (64 units per sample); exp overflow gives 1, underflow -1.
 __device__ __forceinline__ float act_relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+__device__ __forceinline__ float act_tanh(float x) { return 1.f - 2.f * __builtin_amdgcn_rcpf(__expf(2.f * x) + 1.f); }
+__device__ __forceinline__ float act(float x, bool th) { return th ? act_tanh(x) : act_relu(x); }
 
 // ---- features --------------------------------------------------------------
 
@@ -258,11 +262,12 @@ __device__ __forceinline__ float mlp_wave(const MlpPacked &P, const bf16x8 &x, i
     }
     const floatx4 a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.w1[0][lane], x, i0, 0, 0, 0);
     const floatx4 a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P.w1[1][lane], x, i1, 0, 0, 0);
+    const bool th = P.act != 0.f;
     bf16x8 hb;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        hb[i] = (__bf16)act_relu(a0[i]);
-        hb[4 + i] = (__bf16)act_relu(a1[i]);
+        hb[i] = (__bf16)act(a0[i], th);
+        hb[4 + i] = (__bf16)act(a1[i], th);
     }
     if (h1out) {
 #pragma unroll
@@ -278,7 +283,7 @@ __device__ __forceinline__ float mlp_wave(const MlpPacked &P, const bf16x8 &x, i
     float acc = 0.f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const float u0 = act_relu(c0[i]), u1 = act_relu(c1[i]);
+        const float u0 = act(c0[i], th), u1 = act(c1[i], th);
         if (h2out) { (*h2out)[i] = u0; (*h2out)[4 + i] = u1; }
         acc += P.w3[4 * g + i] * u0 + P.w3[16 + 4 * g + i] * u1;
     }
@@ -387,8 +392,10 @@ constexpr int BW_SAMPLES = 64;
 __global__ void __launch_bounds__(256) k_mlp_backward(const float *__restrict__ prm, const float *__restrict__ X,
                                                       const float *__restrict__ H1, const float *__restrict__ H2,
                                                       const float *__restrict__ Y, const uint8_t *__restrict__ T,
-                                                      int n, float pos_weight, float *__restrict__ grad)
+                                                      int n, float pos_weight, float *__restrict__ grad, int tanh_act)
 {
+    // derivative of the activation from its output h: ReLU 1 if h > 0, tanh 1 - h^2
+    auto dact = [&](float h) { return tanh_act ? 1.f - h * h : (h > 0.f ? 1.f : 0.f); };
     const float *W1 = prm, *W2 = prm + 1056, *w3 = prm + 2112;
     float *gW1 = grad, *gb1 = grad + 1024, *gW2 = grad + 1056, *gb2 = grad + 2080, *gw3 = grad + 2112, *gb3 = grad + 2144;
     __shared__ float sW2[32][33];
@@ -416,8 +423,7 @@ __global__ void __launch_bounds__(256) k_mlp_backward(const float *__restrict__ 
     __syncthreads();
     for (int q = threadIdx.x; q < BW_SAMPLES * 32; q += blockDim.x) {
         const int a = q / 32, j = q % 32;
-        const float h = h2s[a][j];
-        dh2[a][j] = h > 0.f ? dzs[a] * w3[j] : 0.f;
+        dh2[a][j] = dzs[a] * w3[j] * dact(h2s[a][j]);
     }
     __syncthreads();
     for (int q = threadIdx.x; q < BW_SAMPLES * 32; q += blockDim.x) {
@@ -425,8 +431,7 @@ __global__ void __launch_bounds__(256) k_mlp_backward(const float *__restrict__ 
         float acc = 0.f;
 #pragma unroll 8
         for (int j = 0; j < 32; ++j) acc += dh2[a][j] * sW2[j][kk];
-        const float h = h1s[a][kk];
-        dh1[a][kk] = h > 0.f ? acc : 0.f;
+        dh1[a][kk] = acc * dact(h1s[a][kk]);
     }
     __syncthreads();
     // parameter gradients: sum over the block's samples, one atomic per entry
@@ -478,7 +483,7 @@ __global__ void k_adam(float *__restrict__ prm, const float *__restrict__ g, flo
     prm[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-__global__ void k_pack(const float *__restrict__ prm, MlpPacked *__restrict__ P)
+__global__ void k_pack(const float *__restrict__ prm, MlpPacked *__restrict__ P, float act)
 {
     const float *W1 = prm, *b1 = prm + 1024, *W2 = prm + 1056, *b2 = prm + 2080, *w3 = prm + 2112, *b3 = prm + 2144;
     const int t = threadIdx.x;   // 128 threads: half b = t / 64, lane = t % 64
@@ -496,7 +501,10 @@ __global__ void k_pack(const float *__restrict__ prm, MlpPacked *__restrict__ P)
         P->b2[t] = b2[t];
         P->w3[t] = w3[t];
     }
-    if (t == 0) P->b3 = b3[0];
+    if (t == 0) {
+        P->b3 = b3[0];
+        P->act = act;
+    }
 }
 
 // Block = 64 x 4 columns marching a chunk of DIL_ZC planes; the own column
@@ -553,11 +561,18 @@ extern "C" {
 
 size_t acx_mlp_packed_bytes(void) { return sizeof(MlpPacked); }
 
-int acx_mlp_pack(const float *params, void *packed, void *stream)
+int acx_mlp_pack_act(const float *params, int activation, void *packed, void *stream)
 {
     if (!params || !packed) return set_error(ACX_EINVAL, "mlp pack: null pointer");
-    hipLaunchKernelGGL(k_pack, dim3(1), dim3(128), 0, (hipStream_t)stream, params, (MlpPacked *)packed);
+    if (activation != ACX_MLP_RELU && activation != ACX_MLP_TANH) return set_error(ACX_EINVAL, "mlp pack: activation");
+    hipLaunchKernelGGL(k_pack, dim3(1), dim3(128), 0, (hipStream_t)stream, params, (MlpPacked *)packed,
+                       activation == ACX_MLP_TANH ? 1.f : 0.f);
     return hip_check(hipGetLastError(), "mlp pack");
+}
+
+int acx_mlp_pack(const float *params, void *packed, void *stream)
+{
+    return acx_mlp_pack_act(params, ACX_MLP_RELU, packed, stream);
 }
 
 int acx_features_f32(const acx_grid *g, const float *U, float *X, void *stream)
@@ -603,11 +618,18 @@ int acx_mlp_forward(const void *packed, const float *X, int n, float *Y, float *
 int acx_mlp_backward(const float *params, const float *X, const float *H1, const float *H2, const float *Y,
                      const uint8_t *T, int n, float pos_weight, float *grad, void *stream)
 {
+    return acx_mlp_backward_act(params, ACX_MLP_RELU, X, H1, H2, Y, T, n, pos_weight, grad, stream);
+}
+
+int acx_mlp_backward_act(const float *params, int activation, const float *X, const float *H1, const float *H2,
+                         const float *Y, const uint8_t *T, int n, float pos_weight, float *grad, void *stream)
+{
     if (!params || !X || !H1 || !H2 || !Y || !T || !grad || n <= 0) return set_error(ACX_EINVAL, "mlp backward: bad arguments");
+    if (activation != ACX_MLP_RELU && activation != ACX_MLP_TANH) return set_error(ACX_EINVAL, "mlp backward: activation");
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_fill, dim3((ACX_MLP_NPARAM + 255) / 256), dim3(256), 0, s, grad, ACX_MLP_NPARAM, 0.f);
     hipLaunchKernelGGL(k_mlp_backward, dim3((n + BW_SAMPLES - 1) / BW_SAMPLES), dim3(256), 0, s, params, X, H1, H2, Y,
-                       T, n, pos_weight, grad);
+                       T, n, pos_weight, grad, activation == ACX_MLP_TANH ? 1 : 0);
     return hip_check(hipGetLastError(), "mlp backward");
 }
 

@@ -20,6 +20,8 @@ from . import _check, _ptr, _stream, lib, ACOUSTIC4, Grid, CompensatedState, fie
 NPARAM = 32 * 32 + 32 + 32 * 32 + 32 + 32 + 1
 NFEAT = 19
 OFF = {"w1": 0, "b1": 1024, "w2": 1056, "b2": 2080, "w3": 2112, "b3": 2144}
+# hidden activation codes of acx_mlp_pack_act / acx_mlp_backward_act
+ACT = {"relu": 0, "tanh": 1}
 
 
 def _bind():
@@ -30,6 +32,8 @@ def _bind():
     gp = _C.c_void_p
     for name, args in {
         "acx_mlp_pack": [vp, vp, vp],
+        "acx_mlp_pack_act": [vp, i32, vp, vp],
+        "acx_mlp_backward_act": [vp, i32, vp, vp, vp, vp, vp, i32, f32, vp, vp],
         "acx_features_f32": [gp, vp, vp, vp],
         "acx_weno_labels_f64": [gp, vp, dbl, vp, vp, vp],
         "acx_mlp_forward": [vp, vp, i32, vp, vp, vp, vp],
@@ -53,25 +57,40 @@ def _g(grid):
     return g, _C.cast(_C.pointer(g), _C.c_void_p)
 
 
-def init_params(seed=0):
-    """He-uniform weights (ReLU layers), zero biases (fp32 numpy, NPARAM)."""
+def _act(act):
+    if act not in ACT:
+        raise ValueError(f"activation must be one of {sorted(ACT)}, not {act!r}")
+    return ACT[act]
+
+
+def init_params(seed=0, act="relu"):
+    """Zero biases and uniform weights (fp32 numpy, NPARAM): He limits
+    sqrt(6 / fan_in) for ReLU layers, Glorot sqrt(6 / (fan_in + fan_out))
+    for tanh layers."""
+    _act(act)
     rng = np.random.default_rng(seed)
     p = np.zeros(NPARAM, np.float32)
-    lim1 = math.sqrt(6.0 / NFEAT)
+    if act == "tanh":
+        lim1, lim2, lim3 = math.sqrt(6.0 / (NFEAT + 32)), math.sqrt(6.0 / 64), math.sqrt(6.0 / 33)
+    else:
+        lim1, lim2 = math.sqrt(6.0 / NFEAT), math.sqrt(6.0 / 32)
+        lim3 = lim2
     w1 = rng.uniform(-lim1, lim1, (32, 32)).astype(np.float32)
     w1[:, NFEAT:] = 0.0
     p[OFF["w1"]:OFF["w1"] + 1024] = w1.ravel()
-    lim2 = math.sqrt(6.0 / 32)
     p[OFF["w2"]:OFF["w2"] + 1024] = rng.uniform(-lim2, lim2, 1024)
-    p[OFF["w3"]:OFF["w3"] + 32] = rng.uniform(-lim2, lim2, 32)
+    p[OFF["w3"]:OFF["w3"] + 32] = rng.uniform(-lim3, lim3, 32)
     return p
 
 
-def pack(params_dev, stream=None):
+def pack(params_dev, stream=None, act="relu"):
+    """MFMA operand layout of the parameters; it carries the hidden
+    activation, which forward, the fused indicator and the stepper use."""
     import torch
     L = _bind()
     buf = torch.empty(L.acx_mlp_packed_bytes(), dtype=torch.uint8, device="cuda")
-    _check(L.acx_mlp_pack(_ptr(params_dev, torch.float32), _C.c_void_p(buf.data_ptr()), _stream(stream)))
+    _check(L.acx_mlp_pack_act(_ptr(params_dev, torch.float32), _act(act), _C.c_void_p(buf.data_ptr()),
+                              _stream(stream)))
     return buf
 
 
@@ -109,13 +128,14 @@ def forward(packed, X, keep=False, stream=None):
     return (Y, H1, H2) if keep else Y
 
 
-def backward(params_dev, X, H1, H2, Y, T, pos_weight, stream=None):
+def backward(params_dev, X, H1, H2, Y, T, pos_weight, stream=None, act="relu"):
     import torch
     L = _bind()
     G = torch.empty(NPARAM, dtype=torch.float32, device="cuda")
-    _check(L.acx_mlp_backward(_ptr(params_dev, torch.float32), _ptr(X, torch.float32), _ptr(H1, torch.float32),
-                              _ptr(H2, torch.float32), _ptr(Y, torch.float32), _C.c_void_p(T.data_ptr()),
-                              X.shape[0], pos_weight, _ptr(G, torch.float32), _stream(stream)))
+    _check(L.acx_mlp_backward_act(_ptr(params_dev, torch.float32), _act(act), _ptr(X, torch.float32),
+                                  _ptr(H1, torch.float32), _ptr(H2, torch.float32), _ptr(Y, torch.float32),
+                                  _C.c_void_p(T.data_ptr()), X.shape[0], pos_weight, _ptr(G, torch.float32),
+                                  _stream(stream)))
     return G
 
 
@@ -172,12 +192,13 @@ def make_dataset(n_snapshots=8, N=64, seed0=100, steps_between=12, thr=0.05):
     return torch.cat(Xs), torch.cat(Ts)
 
 
-def train(X, T, epochs=8, batch=8192, lr=3e-3, seed=0, pos_weight=None, log=None):
-    """Trains the indicator with the library's kernels; returns fp32 params (GPU)."""
+def train(X, T, epochs=8, batch=8192, lr=3e-3, seed=0, pos_weight=None, log=None, act="relu"):
+    """Trains the indicator with the library's kernels; returns fp32 params
+    (GPU), to be packed with the same activation."""
     import torch
     L = _bind()
     n = X.shape[0]
-    p = torch.from_numpy(init_params(seed)).cuda()
+    p = torch.from_numpy(init_params(seed, act)).cuda()
     m = torch.zeros_like(p)
     v = torch.zeros_like(p)
     if pos_weight is None:
@@ -191,9 +212,9 @@ def train(X, T, epochs=8, batch=8192, lr=3e-3, seed=0, pos_weight=None, log=None
         for b0 in range(0, n, batch):
             idx = perm[b0:b0 + batch]
             xb, tb = X[idx].contiguous(), T[idx].contiguous()
-            packed = pack(p)
+            packed = pack(p, act=act)
             Y, H1, H2 = forward(packed, xb, keep=True)
-            G = backward(p, xb, H1, H2, Y, tb, pos_weight)
+            G = backward(p, xb, H1, H2, Y, tb, pos_weight, act=act)
             step += 1
             _check(L.acx_mlp_adam(_ptr(p, torch.float32), _ptr(G, torch.float32), _ptr(m, torch.float32),
                                   _ptr(v, torch.float32), step, lr, 0.9, 0.999, 1e-8, _stream()))

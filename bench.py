@@ -330,23 +330,30 @@ def variant_fp32c(torch, acx, M, steps, fp64_value, err_steps=200):
     return out
 
 
-def train_indicator(torch, acx):
+def train_indicator(torch, acx, acts=("relu",)):
     """The troubled-cell MLP trained on the GPU by the library's own kernels:
     8 solver-labelled snapshots of 64^3 steep-pulse runs (2.1e6 cells, 8.4e6
     (cell, component) samples), labelled positive where the eight normalised
-    CWENO4 weights leave the linear weights, sum_k |w_k - gamma_k| > 0.05."""
+    CWENO4 weights leave the linear weights, sum_k |w_k - gamma_k| > 0.05.
+    One network per hidden activation in `acts`, on the same samples."""
     from acoustic_petsc_amd import learn
     t0 = time.perf_counter()
     X, T = learn.make_dataset(n_snapshots=8, N=64, seed0=100)
-    p = learn.train(X, T, epochs=6, batch=8192, lr=3e-3, seed=1)
-    packed = learn.pack(p)
     torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    rec, prec, frac = learn.evaluate(packed, X, T, 0.3)
-    info = {"samples": int(X.shape[0]), "positive_frac": float(T.float().mean()), "train_s": el,
-            "recall": rec, "precision": prec, "flagged_frac": frac}
+    t_data = time.perf_counter() - t0
+    out = {}
+    for act in acts:
+        t0 = time.perf_counter()
+        p = learn.train(X, T, epochs=6, batch=8192, lr=3e-3, seed=1, act=act)
+        packed = learn.pack(p, act=act)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        rec, prec, frac = learn.evaluate(packed, X, T, 0.3)
+        out[act] = (packed, {"activation": act, "samples": int(X.shape[0]),
+                             "positive_frac": float(T.float().mean()), "dataset_s": t_data, "train_s": el,
+                             "recall": rec, "precision": prec, "flagged_frac": frac})
     del X, T
-    return packed, info
+    return out
 
 
 def variant_learned(torch, acx, sizes, steps, var, fp64_value):
@@ -354,7 +361,8 @@ def variant_learned(torch, acx, sizes, steps, var, fp64_value):
     flagged cells, linear reconstruction elsewhere) against the FP64 nonlinear
     stepper on a held-out steep pulse, nonlinear weights gamma = 0.85/0.85."""
     from acoustic_petsc_amd import learn
-    packed, info = train_indicator(torch, acx)
+    nets = train_indicator(torch, acx, ("relu", "tanh"))
+    packed, info = nets["relu"]
     res = {"indicator_training": info}
     for M in sizes:
         g = acx.Grid(n=(M, M, M), h=1.0 / M, system=acx.ACOUSTIC4, gamma_hi=0.85, gamma_lo=0.85)
@@ -412,6 +420,32 @@ def variant_learned(torch, acx, sizes, steps, var, fp64_value):
             "fp64_weno_value": v64, "fp64_linear_value": vl64,
             "fp32_all_weno_value": cells * STAGES * max(2, k // 4) / tfull,
             "speedup_vs_fp64_weno": v / v64, "speedup_vs_fp64_linear": v / vl64}
+    # the same stepper with the tanh network, largest grid only
+    M = sizes[-1]
+    g = acx.Grid(n=(M, M, M), h=1.0 / M, system=acx.ACOUSTIC4, gamma_hi=0.85, gamma_lo=0.85)
+    U0 = torch.from_numpy(learn.steep_pulse_field(g, 4242))
+    dt = acx.cfl_dt(g, 0.4)
+    k = max(2, min(20, steps))
+    ref = U0.cuda()
+    w = (torch.empty_like(ref), torch.empty_like(ref))
+    for _ in range(k + 1):
+        acx.ssp_rks3_step(g, ref, dt, w)
+    del w
+    tpacked, tinfo = nets["tanh"]
+    sel = learn.SelectiveStepper(g, tpacked, U0.cuda(), thr=0.3, refresh=12, dilate=3)
+    sel.step(dt)
+    tsel, _ = time_steps(torch, lambda: sel.step(dt), k, 0)
+    rel, mx = acx.diff_f32c(sel.state.hi, sel.state.lo, ref)
+    t_ind, _ = time_steps(torch, lambda: sel._mlp_flags(), 3, 1)
+    v = M ** 3 * STAGES * k / tsel
+    res["tanh_activation"] = {
+        "training": tinfo, "grid": [M, M, M], "value": v, "ms_per_step": 1e3 * tsel / k,
+        "indicator_ms": 1e3 * t_ind / 3, "flagged_frac_last_step": sel.flagged_fraction(),
+        "rel_l2_vs_fp64_weno": rel, "max_abs_vs_fp64_weno": mx,
+        "judge": {"windows_judged": sel.judged_windows, "misses": sel.misses, "windows_rerun": sel.redone_windows},
+        "relu_value": res[f"fp32_learned_{M}"]["value"]}
+    del sel, ref
+    torch.cuda.empty_cache()
     return res
 
 

@@ -12,7 +12,8 @@
  * 4th-order polynomial while full CWENO stays the fallback everywhere else.
  *
  * Network: 19 inputs (18 normalised undivided differences, 6 per axis, plus
- * a log-amplitude term), padded to 32 -> 32 ReLU -> 32 ReLU -> 1 sigmoid.
+ * a log-amplitude term), padded to 32 -> 32 -> 32 -> 1 sigmoid, hidden
+ * activation ReLU (default) or tanh (ACX_MLP_TANH).
  * Inference and the training forward run on MFMA (bf16 operands, fp32
  * accumulation); the backward pass and Adam run in fp32.
  */
@@ -33,10 +34,15 @@ extern "C" {
  * ACX_MLP_NPARAM floats in this order: w1, b1, w2, b2, w3, b3. */
 #define ACX_MLP_NPARAM (32 * 32 + 32 + 32 * 32 + 32 + 32 + 1)
 
+#define ACX_MLP_RELU 0
+#define ACX_MLP_TANH 1
+
 /* Packs the fp32 parameters into the MFMA operand layout (device to device);
- * `packed` needs acx_mlp_packed_bytes() bytes. */
+ * `packed` needs acx_mlp_packed_bytes() bytes and carries the hidden
+ * activation (acx_mlp_pack: ReLU).  Forward and inference use it. */
 size_t acx_mlp_packed_bytes(void);
 int acx_mlp_pack(const float *params, void *packed, void *stream);
+int acx_mlp_pack_act(const float *params, int activation, void *packed, void *stream);
 
 /* Feature vectors of every (cell, component): X[(cell*4 + v)*32 + f] (fp32,
  * zero padded).  U is an ACOUSTIC4 state (fp32, interlaced). */
@@ -58,6 +64,11 @@ int acx_mlp_forward(const void *packed, const float *X, int n, float *Y, float *
  * Uses the bf16-rounded weights and inputs the forward pass used. */
 int acx_mlp_backward(const float *params, const float *X, const float *H1, const float *H2, const float *Y,
                      const uint8_t *T, int n, float pos_weight, float *grad, void *stream);
+/* The same for either activation (acx_mlp_backward: ReLU); the derivative
+ * is taken from the activation H the forward pass kept (ReLU: H > 0, tanh:
+ * 1 - H^2). */
+int acx_mlp_backward_act(const float *params, int activation, const float *X, const float *H1, const float *H2,
+                         const float *Y, const uint8_t *T, int n, float pos_weight, float *grad, void *stream);
 
 /* One Adam step (fp32): params -= lr * mhat / (sqrt(vhat) + eps). */
 int acx_mlp_adam(float *params, const float *grad, float *m, float *v, int step, float lr, float beta1,
