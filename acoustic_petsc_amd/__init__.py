@@ -122,6 +122,7 @@ def lib():
             "acx_sel_mask": [gp, vp, vp, vp],
             "acx_norms_f64": [vp, _C.c_size_t, i32, i32, _C.POINTER(dbl), vp],
             "acx_diff_f32c_f64": [vp, vp, vp, _C.c_size_t, _C.POINTER(dbl), vp],
+            "acx_vec_copy_f64": [vp, vp, _C.c_size_t, vp],
             "acx_energy_f64": [gp, vp, _C.POINTER(dbl), vp],
             "acx_ctcs_update_f64": [_C.POINTER(_Ctcs), vp, vp, vp, vp, i32, _C.POINTER(dbl), vp],
             "acx_field_gaussians": [gp, _C.c_uint64, i32, dbl, dbl, dbl, dbl, vp],
@@ -287,6 +288,16 @@ def norms(X, comp=-1, nvar=None, stream=None):
     out = (_C.c_double * 2)()
     _check(lib().acx_norms_f64(_ptr(X, torch.float64), X.numel() // nv, nv, comp, out, _stream(stream)))
     return out[0], out[1]
+
+
+def vec_copy(X, Y, stream=None):
+    """Y[:] = X (VecCopy) for FP64 device tensors of equal size, with the
+    library's nontemporal copy kernel."""
+    torch = _torch()
+    if X.numel() != Y.numel():
+        raise ValueError(f"vec_copy: sizes differ ({X.numel()} and {Y.numel()})")
+    _check(lib().acx_vec_copy_f64(_ptr(X, torch.float64), _ptr(Y, torch.float64), X.numel(), _stream(stream)))
+    return Y
 
 
 def diff_f32c(hi, lo, ref, stream=None):

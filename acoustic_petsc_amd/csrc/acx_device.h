@@ -168,6 +168,19 @@ __device__ __forceinline__ void store_cell_nt(T *__restrict__ base, size_t idx, 
 }
 
 
+// Exchanges a's lanes 32-63 with b's lanes 0-31 (v_permlane32_swap, two
+// dwords per double): afterwards a holds [a of lanes 0-31, b of lanes 0-31]
+// and b holds [a of lanes 32-63, b of lanes 32-63].  Every lane of the
+// wavefront must execute it.
+__device__ __forceinline__ void swap_halves32(double &a, double &b)
+{
+    const unsigned long long ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+    a = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+    b = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+
 // Linear-scheme face stencils folded into 7-point cell stencils.
 // With the shipped weights (gamma_hi = 1, gamma_lo = 0) the CWENO4 face
 // averages are q_L(i+1/2) = (q_{i-2} - 6q_{i-1} + 20q_i + 10q_{i+1} - q_{i+2})/24

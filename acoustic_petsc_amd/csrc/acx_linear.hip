@@ -40,14 +40,16 @@ k_linear(LinArgs p)
 
 // LDS-DMA march (acx_linear_core.h): FP64 ACOUSTIC4 RHS / AXPY stages on
 // all-periodic grids; one block of 512 threads per CU (the ring is 160.5 KB).
-template <int EPI>
+// RS: row-complete result stores, taken when the launch needs more than one
+// round of workgroups.
+template <int EPI, bool RS>
 __global__ void __launch_bounds__(DmaPlan::BX *DmaPlan::BY) k_linear_dma(LinArgs p)
 {
     __shared__ __attribute__((aligned(16))) unsigned char ring[DmaPlan::NSLOT * DmaPlan::SLOT];
     const int tile = xcd_logical_tile(blockIdx.x, gridDim.x);
     const int x0 = (tile % p.ntx) * DmaPlan::BX, y0 = ((tile / p.ntx) % p.nty) * DmaPlan::BY;
     const int zb = (tile / (p.ntx * p.nty)) * p.zc;
-    linear_tile_dma<EPI>(p, x0, y0, zb, ring);
+    linear_tile_dma<EPI, RS>(p, x0, y0, zb, ring);
 }
 
 template <int EPI>
@@ -56,19 +58,22 @@ static int launch_linear_dma(const LinArgs &a, hipStream_t s)
     LinArgs b = a;
     b.ntx = (a.nx + DmaPlan::BX - 1) / DmaPlan::BX;
     b.nty = (a.ny + DmaPlan::BY - 1) / DmaPlan::BY;
-    if (b.zc <= 0) {
-        static int per_cu = 0;
-        if (!per_cu) {
-            int n = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(&k_linear_dma<EPI>),
-                                                             DmaPlan::BX * DmaPlan::BY, 0) != hipSuccess || n <= 0)
-                n = 1;
-            per_cu = n;
-        }
-        b.zc = auto_zc(b.ntx, b.nty, a.nz, device_cus() * per_cu, 10);
+    static int per_cu = 0;
+    if (!per_cu) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(&k_linear_dma<EPI, false>),
+                                                         DmaPlan::BX * DmaPlan::BY, 0) != hipSuccess || n <= 0)
+            n = 1;
+        per_cu = n;
     }
+    const int slots = device_cus() * per_cu;
+    if (b.zc <= 0) b.zc = auto_zc(b.ntx, b.nty, a.nz, slots, 10);
     b.ntz = (a.nz + b.zc - 1) / b.zc;
-    hipLaunchKernelGGL(k_linear_dma<EPI>, dim3(b.ntx * b.nty * b.ntz), dim3(DmaPlan::BX, DmaPlan::BY), 0, s, b);
+    const long long wgs = (long long)b.ntx * b.nty * b.ntz;
+    if (wgs > slots)
+        hipLaunchKernelGGL((k_linear_dma<EPI, true>), dim3((unsigned)wgs), dim3(DmaPlan::BX, DmaPlan::BY), 0, s, b);
+    else
+        hipLaunchKernelGGL((k_linear_dma<EPI, false>), dim3((unsigned)wgs), dim3(DmaPlan::BX, DmaPlan::BY), 0, s, b);
     return hip_check(hipGetLastError(), "k_linear_dma launch");
 }
 

@@ -42,6 +42,45 @@ __device__ __forceinline__ void block_reduce(double &s, double &m)
     }
 }
 
+// ---- vector copy ------------------------------------------------------------
+// Y = X with nontemporal 16-byte loads and stores: each thread issues its
+// CP_UNROLL loads before the first store, so a wave keeps 4 KiB in flight;
+// a block covers CP_THREADS * CP_UNROLL pairs.  k_copy1 is the scalar form
+// for pointers off 16-byte alignment and for the odd last element.
+constexpr int CP_THREADS = 256, CP_UNROLL = 4;
+
+__global__ void __launch_bounds__(CP_THREADS) k_copy2(const acx_dbl2 *__restrict__ X, acx_dbl2 *__restrict__ Y, size_t n2)
+{
+    const size_t base = (size_t)blockIdx.x * (CP_THREADS * CP_UNROLL) + threadIdx.x;
+    acx_dbl2 v[CP_UNROLL];
+#pragma unroll
+    for (int u = 0; u < CP_UNROLL; ++u) {
+        const size_t i = base + (size_t)u * CP_THREADS;
+        if (i < n2) v[u] = __builtin_nontemporal_load(X + i);
+    }
+#pragma unroll
+    for (int u = 0; u < CP_UNROLL; ++u) {
+        const size_t i = base + (size_t)u * CP_THREADS;
+        if (i < n2) __builtin_nontemporal_store(v[u], Y + i);
+    }
+}
+
+__global__ void __launch_bounds__(CP_THREADS) k_copy1(const double *__restrict__ X, double *__restrict__ Y, size_t n)
+{
+    const size_t base = (size_t)blockIdx.x * (CP_THREADS * CP_UNROLL) + threadIdx.x;
+    double v[CP_UNROLL];
+#pragma unroll
+    for (int u = 0; u < CP_UNROLL; ++u) {
+        const size_t i = base + (size_t)u * CP_THREADS;
+        if (i < n) v[u] = X[i];
+    }
+#pragma unroll
+    for (int u = 0; u < CP_UNROLL; ++u) {
+        const size_t i = base + (size_t)u * CP_THREADS;
+        if (i < n) Y[i] = v[u];
+    }
+}
+
 // mode 0: x = X[i] (component filter); mode 1: x = (hi + lo) - B, also sums B^2
 __global__ void __launch_bounds__(RED_THREADS)
 k_reduce_pass1(const double *__restrict__ X, size_t n, int nvar, int comp, double *__restrict__ part)
@@ -312,6 +351,29 @@ int acx_norms_f64(const double *X, size_t ncells, int nvar, int comp, double *ou
     out[0] = r[0];
     out[1] = r[1];
     return ACX_OK;
+}
+
+int acx_vec_copy_f64(const double *X, double *Y, size_t n, void *stream)
+{
+    if (n == 0) return ACX_OK;
+    if (!X || !Y) return set_error(ACX_EINVAL, "vec copy: null pointer");
+    if ((X < Y + n) && (Y < X + n) && X != Y) return set_error(ACX_EINVAL, "vec copy: overlapping arrays");
+    if (X == Y) return ACX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t per_block = (size_t)CP_THREADS * CP_UNROLL;
+    const bool aligned = (((uintptr_t)X | (uintptr_t)Y) & 15) == 0;
+    const size_t nv = aligned ? n / 2 : n;                 // elements of the main launch
+    const size_t blocks = (nv + per_block - 1) / per_block;
+    if (blocks > 0x7fffffffULL) return set_error(ACX_EINVAL, "vec copy: %zu elements is too many", n);
+    if (blocks) {
+        if (aligned)
+            hipLaunchKernelGGL(k_copy2, dim3((unsigned)blocks), dim3(CP_THREADS), 0, s, (const acx_dbl2 *)X,
+                               (acx_dbl2 *)Y, nv);
+        else
+            hipLaunchKernelGGL(k_copy1, dim3((unsigned)blocks), dim3(CP_THREADS), 0, s, X, Y, nv);
+    }
+    if (aligned && (n & 1)) hipLaunchKernelGGL(k_copy1, dim3(1), dim3(CP_THREADS), 0, s, X + n - 1, Y + n - 1, (size_t)1);
+    return hip_check(hipGetLastError(), "vec copy");
 }
 
 int acx_diff_f32c_f64(const float *HI, const float *LO, const double *B, size_t n, double *out, void *stream)

@@ -506,7 +506,8 @@ PetscErrorCode VecCopy(Vec x, Vec y)
     if (x->n != y->n) return PetscLiteError(60, "VecCopy: size mismatch");
     PetscErrorCode e;
     if ((e = to_dev(x))) return e;
-    HIPCALL(hipMemcpy(y->dev, x->dev, x->n * sizeof(double), hipMemcpyDeviceToDevice));
+    const int rc = acx_vec_copy_f64(x->dev, y->dev, x->n, nullptr);
+    if (rc) return PetscLiteError(76 + rc, "VecCopy: %s", acx_last_error());
     y->state = VEC_DEV;
     return 0;
 }

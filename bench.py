@@ -12,12 +12,23 @@ seeded Gaussian pressure pulses (amplitudes U[1e-4,1e-3], sigma
 U[0.02,0.08]), zero velocity; shipped linear weights gamma = (1, 0).  The
 state is resident in HBM before timing starts.
 
+Beside the headline line's roofline: the read + write rate of state-sized
+device copies on the same stream (the library's nontemporal VecCopy kernel
+and torch's copy_), which bounds what a stage that streams the state can
+reach on the box it runs on.
+
 Also reported ("variants"): the nonlinear CWENO4 (gamma = 0.85/0.85) FP64
-kernel on the same workload, and the FP32 compensated path at 384^3 on the
-band-limited noise + pulse field, with its error against an FP64 run.
+kernel on the same workload; the FP32 compensated path at 384^3 with its
+error against FP64; the learned troubled-cell indicator's selective FP32
+stepper at 256^3 and 384^3 (ReLU network, and the tanh network at 384^3)
+against FP64 full CWENO4; the 48-problem ensemble; the 2D acoustic bubble
+(eager and graph-replayed); the CTCS leapfrog sibling; the central WAVE2
+system.  A CPU baseline (the C oracle on the host cores, a bounded sample)
+is timed unless --no-cpu.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
-                       [--variants all|none] [--no-cpu]
+                       [--variants all|none|name,...] [--learned-sizes 256,384]
+                       [--no-cpu] [--calib-copy]
 """
 import argparse
 import json
@@ -116,6 +127,23 @@ def settle(torch, step, batch=10, tol=0.01, max_s=3.0):
         if time.perf_counter() > t_end:
             return n, (last, t)
         last = t
+
+
+def note(msg):
+    """Progress line on stderr (stdout carries only the JSON result)."""
+    print(f"[bench] {time.strftime('%H:%M:%S')} {msg}", file=sys.stderr, flush=True)
+
+
+def copy_ceiling(torch, acx, U, W, reps=20):
+    """Read + write rate (GB/s) of state-sized device copies, timed like the
+    stages on the launch stream: the library's nontemporal VecCopy kernel
+    (acx_vec_copy_f64) and torch's copy_.  A stage kernel that reads and
+    writes the state once cannot beat the faster of the two by much."""
+    nbytes = 2 * U.numel() * U.element_size()
+    t_acx, _ = time_steps(torch, lambda: acx.vec_copy(U, W), reps, 5)
+    t_torch, _ = time_steps(torch, lambda: W.copy_(U), reps, 5)
+    return {"bytes_per_copy": nbytes, "acx_vec_copy_GBs": nbytes * reps / t_acx / 1e9,
+            "torch_copy_GBs": nbytes * reps / t_torch / 1e9}
 
 
 def _host_cpu_model():
@@ -531,6 +559,7 @@ def main():
         del tmp
 
     headline_step = lambda: acx.ssp_rks3_step(grid, U, dt, work)
+    note(f"headline {N}^3: settling")
     settle_steps, settle_pair = settle(torch, headline_step)
     per_step = {}
     t_gpu, wall = time_steps(torch, headline_step, args.steps, args.warmup, per_step)
@@ -538,7 +567,11 @@ def main():
     value = cells * STAGES * args.steps / t_gpu
     launch_s = t_gpu / (STAGES * args.steps)
     achieved = BYTES_FP64_AC4 * cells / launch_s / 1e9
+    note(f"headline {N}^3: {1e3 * t_gpu / args.steps:.4f} ms per step after {settle_steps} settle steps")
     traffic = traffic_from_profiles(N)
+    copy = copy_ceiling(torch, acx, U, work[0])
+    note(f"copy ceiling: {copy['acx_vec_copy_GBs']:.0f} GB/s (acx_vec_copy), {copy['torch_copy_GBs']:.0f} GB/s (torch)")
+    copy_peak = max(copy["acx_vec_copy_GBs"], copy["torch_copy_GBs"])
     out = {
         "metric": METRIC,
         "value": value,
@@ -569,6 +602,8 @@ def main():
                      "traffic_source": f"profiles/k_linear_fp64_ac4_{N}_pmc.md (FETCH_SIZE/WRITE_SIZE "
                                        "of a rocprofv3 --pmc run, not measured in this run)"
                      if traffic is not None else None,
+                     "copy_ceiling": copy,
+                     "frac_of_copy": achieved / copy_peak,
                      "kernel": "k_linear<double, ACOUSTIC4> (fused RHS + stage)",
                      "bytes_per_cell_update": BYTES_FP64_AC4,
                      "bytes_per_launch": BYTES_FP64_AC4 * cells},
@@ -581,25 +616,34 @@ def main():
             raise SystemExit(f"unknown variants {sorted(unknown)}; choose from {VARIANTS}")
         var = {}
         if "fp64_cweno" in want:
+            note("variant fp64_cweno")
             var["fp64_cweno_g085"] = variant_fp64_cweno(torch, acx, N, U0, work, dt, args.steps)
         if "fp32c" in want:
+            note("variant fp32c")
             var["fp32c_linear_384"] = variant_fp32c(torch, acx, 384 if N == 256 else N, args.steps, value)
         del work, U
         torch.cuda.empty_cache()
         if "learned" in want:
+            note("variant learned")
             var.update(variant_learned(torch, acx, args.learned_sizes, args.steps, var, value))
         if "ensemble" in want:
+            note("variant ensemble")
             var["ensemble_48"] = variant_ensemble(torch, acx, args.steps)
         if "2d" in want:
+            note("variant 2d")
             var["acoustic2d_256"] = variant_2d(torch, acx)
         if "ctcs" in want:
+            note("variant ctcs")
             var["ctcs_leapfrog"] = variant_ctcs(torch, acx)
         if "wave2" in want:
+            note("variant wave2")
             var["wave2_256"] = variant_wave2(torch, acx)
         out["variants"] = var
 
     if not args.no_cpu:
+        note("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(acx)
+    note("done")
     print(json.dumps(out), flush=True)
 
 

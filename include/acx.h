@@ -106,6 +106,12 @@ int acx_ssp_rks3_step_f32c(const acx_grid *g, const float *HI, const float *LO, 
  * sums live in a buffer taken per call on `stream` (stream-ordered), so
  * calls on different streams do not share scratch. */
 int acx_norms_f64(const double *X, size_t ncells, int nvar, int comp, double *out, void *stream);
+/* Y = X for n doubles, stream-ordered (VecCopy, used by the SSP stages of
+ * PETSc's TSSSP, which the reference selects at 3d/weno4/plane_wave/
+ * hype.c:180-182): nontemporal 16-byte loads and stores, the state-sized
+ * copy rate the stage kernels are compared with.  Overlapping distinct
+ * arrays are refused (ACX_EINVAL); X == Y is a no-op. */
+int acx_vec_copy_f64(const double *X, double *Y, size_t n, void *stream);
 /* relative L2 of (A-B) against B and max |A-B|, FP64 accumulation; A is
  * float hi (+ lo, may be NULL), B double. */
 int acx_diff_f32c_f64(const float *HI, const float *LO, const double *B, size_t n,

@@ -200,3 +200,28 @@ def test_reductions_on_two_streams(acx, torch_gpu):
     tb = threading.Thread(target=run, args=("b", B, s2))
     ta.start(); tb.start(); ta.join(); tb.join()
     assert out["a"] == ref_a and out["b"] == ref_b
+
+
+@pytest.mark.parametrize("n,off_x,off_y", [(1, 0, 0), (7, 0, 0), (1 << 20, 0, 0), (1 << 20, 1, 0),
+                                           (12345, 0, 1), (4097, 1, 1), (262147, 0, 0)])
+def test_vec_copy_matches(acx, torch_gpu, n, off_x, off_y):
+    """VecCopy: bit-exact for even and odd sizes and for arrays 8 bytes off
+    16-byte alignment (the scalar form); the bytes after Y's end stay."""
+    torch = torch_gpu
+    print(f"[case] vec_copy n={n} off=({off_x},{off_y})", flush=True)
+    rng = np.random.default_rng(n)
+    src = torch.from_numpy(rng.standard_normal(n + 2)).cuda()
+    dst = torch.full((n + 3,), -7.0, dtype=torch.float64, device="cuda")
+    X, Y = src[off_x:off_x + n], dst[off_y:off_y + n]
+    acx.vec_copy(X, Y)
+    d = dst.cpu().numpy()
+    assert np.array_equal(d[off_y:off_y + n], src.cpu().numpy()[off_x:off_x + n])
+    assert np.all(d[:off_y] == -7.0) and np.all(d[off_y + n:] == -7.0)
+
+
+def test_vec_copy_refuses_overlap(acx, torch_gpu):
+    torch = torch_gpu
+    a = torch.zeros(100, dtype=torch.float64, device="cuda")
+    with pytest.raises(acx.AcxError):
+        acx.vec_copy(a[:50], a[10:60])
+    acx.vec_copy(a, a)            # same array: nothing to do

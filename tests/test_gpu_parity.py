@@ -231,3 +231,31 @@ def test_z_chunk_does_not_change_results(gamma, acx, torch_gpu, monkeypatch):
     for U, hi, lo in results[1:]:
         assert torch.equal(U, results[0][0])
         assert torch.equal(hi, results[0][1]) and torch.equal(lo, results[0][2])
+
+
+def test_multi_round_launch_stores_rows_whole(acx, torch_gpu, monkeypatch):
+    """FP64 linear stages of a launch with more workgroups than the GPU holds
+    at once store each output row whole after a lane exchange; one that fits
+    stores per cell.  Both give the same bits, here on a grid whose last
+    column and row tiles are partial (90 = 2 x 32 + 26, 70 = 4 x 16 + 6):
+    z-chunks of 2 and 3 planes make 480 and 330 workgroups, 64 makes 15."""
+    torch = torch_gpu
+    n = (90, 70, 64)
+    grid = acx.Grid(n=n, h=1.0 / 90, system=acx.ACOUSTIC4)
+    U0 = torch.from_numpy(acx.field_gaussians(grid, 909, npulse=3, amp=(1e-4, 1e-3), sigma=(0.05, 0.1))).cuda()
+    dt = acx.cfl_dt(grid, 0.4)
+    results = []
+    for zc in (64, 2, 3, None):
+        print(f"[case] row stores ACX_ZC_LINEAR={zc}", flush=True)
+        if zc is None:
+            monkeypatch.delenv("ACX_ZC_LINEAR", raising=False)
+        else:
+            monkeypatch.setenv("ACX_ZC_LINEAR", str(zc))
+        U = U0.clone()
+        w = (torch.empty_like(U), torch.empty_like(U))
+        F = acx.rhs(grid, U)
+        for _ in range(2):
+            acx.ssp_rks3_step(grid, U, dt, w)
+        results.append((F, U))
+    for F, U in results[1:]:
+        assert torch.equal(F, results[0][0]) and torch.equal(U, results[0][1])
