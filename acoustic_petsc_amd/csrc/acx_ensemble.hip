@@ -32,8 +32,11 @@ static const EnsConfig kConfigs[ENS_NCONFIG] = {{8, 16}, {8, 32}, {8, 64}, {16, 
                                                 {8, 128}, {16, 128}};
 
 // EPI: EPI_F32C for stages 1-3, EPI_F32CF for the stage that ends the step.
+// Registers capped for 4 waves per SIMD: the final stage with the non-finite
+// watch otherwise takes 133 VGPRs on the 8-row tiles, 3 waves per SIMD, and
+// loses 7 % of the ensemble rate (measured on MI355X).
 template <int BY, int EPI>
-__global__ void __launch_bounds__(ENS_BX *BY)
+__global__ void __launch_bounds__(ENS_BX *BY) __attribute__((amdgpu_waves_per_eu(4)))
 k_ensemble(const EnsProblem *__restrict__ probs, int nprob, int total, unsigned *__restrict__ counter, LinArgs base)
 {
     using Tile = LinTile<float, ACX_ACOUSTIC4, ENS_BX, BY>;
