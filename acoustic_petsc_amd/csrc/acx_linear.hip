@@ -77,6 +77,40 @@ static int launch_linear_dma(const LinArgs &a, hipStream_t s)
     return hip_check(hipGetLastError(), "k_linear_dma launch");
 }
 
+// WAVE2 RHS / AXPY stages on the same march: 80 KB of LDS, two blocks per CU
+// (registers capped for 4 waves per SIMD).
+template <int EPI>
+__global__ void __launch_bounds__(DmaPlan::BX *DmaPlan::BY) __attribute__((amdgpu_waves_per_eu(4)))
+k_linear_dma_w2(LinArgs p)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char ring[DmaPlanW2::NSLOT * DmaPlanW2::SLOT];
+    const int tile = xcd_logical_tile(blockIdx.x, gridDim.x);
+    const int x0 = (tile % p.ntx) * DmaPlan::BX, y0 = ((tile / p.ntx) % p.nty) * DmaPlan::BY;
+    const int zb = (tile / (p.ntx * p.nty)) * p.zc;
+    linear_tile_dma_w2<EPI>(p, x0, y0, zb, ring);
+}
+
+template <int EPI>
+static int launch_linear_dma_w2(const LinArgs &a, hipStream_t s)
+{
+    LinArgs b = a;
+    b.ntx = (a.nx + DmaPlan::BX - 1) / DmaPlan::BX;
+    b.nty = (a.ny + DmaPlan::BY - 1) / DmaPlan::BY;
+    static int per_cu = 0;
+    if (!per_cu) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void *>(&k_linear_dma_w2<EPI>),
+                                                         DmaPlan::BX * DmaPlan::BY, 0) != hipSuccess || n <= 0)
+            n = 1;
+        per_cu = n;
+    }
+    if (b.zc <= 0) b.zc = auto_zc(b.ntx, b.nty, a.nz, device_cus() * per_cu, 10);
+    b.ntz = (a.nz + b.zc - 1) / b.zc;
+    hipLaunchKernelGGL(k_linear_dma_w2<EPI>, dim3((unsigned)(b.ntx * b.nty * b.ntz)), dim3(DmaPlan::BX, DmaPlan::BY),
+                       0, s, b);
+    return hip_check(hipGetLastError(), "k_linear_dma_w2 launch");
+}
+
 // ---------------------------------------------------------------------------
 
 // Automatic z-chunk (auto_zc, acx_api.hip): measured for these kernels (ACOUSTIC4,
@@ -125,6 +159,12 @@ static int launch_linear_t(const LinArgs &a, hipStream_t s)
         // step 0.958 -> 0.905 ms): periodic grids whose plane offsets fit 32 bits
         if (!walls && (size_t)a.nx * a.ny * 4 < 0x7fffffffu)
             return launch_linear_dma<EPI>(a, s);
+    }
+    if constexpr (sizeof(T) == 8 && SYS == ACX_WAVE2 && (EPI == EPI_RHS || EPI == EPI_AXPY)) {
+        // measured: the WAVE2 step 0.485 -> 0.459 ms at 256^3, 1.904 -> 1.760 ms
+        // at 384^3, bit-identical to the halo-ring kernel
+        if (!walls && (size_t)a.nx * a.ny * 2 < 0x7fffffffu)
+            return launch_linear_dma_w2<EPI>(a, s);
     }
     if constexpr (sizeof(T) == 8) {
         return walls ? launch_linear_v<T, SYS, EPI, -2, 1, 32, 16, true, true>(a, s)

@@ -798,4 +798,127 @@ __device__ __forceinline__ void linear_tile_dma(const LinArgs &p, int x0, int y0
     watch_flush(p.nf, nfbad);
 }
 
+// The LDS-DMA march for the FP64 WAVE2 system (u, v: 16 bytes per cell, one
+// slot entry), RHS and AXPY stages on all-periodic grids: 6 slots of 13.4 KB,
+// so two 512-thread blocks fit a CU.  du/dt = v, dv/dt = c^2 Lap(u) with the
+// L7 stencil per axis (3d/weno4/plane_wave/rhs_function.c:326-338 with the
+// shipped linear weights).
+struct DmaPlanW2 {
+    static constexpr int NE = DmaPlan::NE;                       // 836 cells per plane
+    static constexpr int SLOT = NE * 16;                         // 13376 bytes
+    static constexpr int NSLOT = 6;
+    static constexpr int NCHUNK = (NE + 63) / 64;                // 14 wavefront loads per plane
+    static constexpr int MAXJ = (NCHUNK + DmaPlan::NWAVE - 1) / DmaPlan::NWAVE;   // 2
+};
+
+template <int EPI>
+__device__ __forceinline__ void linear_tile_dma_w2(const LinArgs &p, int x0, int y0, int zb, unsigned char *ring)
+{
+    using D = DmaPlan;
+    using W = DmaPlanW2;
+    static_assert(EPI == EPI_RHS || EPI == EPI_AXPY, "LDS-DMA march: RHS and AXPY stages");
+    const double *__restrict__ Q = static_cast<const double *>(p.q);
+    double *__restrict__ OUT = static_cast<double *>(p.out);
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * D::BX + tx;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int ze = min(zb + p.zc, nz);
+    const bool active = (x0 + tx < nx) && (y0 + ty < ny);
+    const size_t plane = (size_t)nx * ny;
+
+    const int nj = wave + D::NWAVE * (W::MAXJ - 1) < W::NCHUNK ? W::MAXJ : W::MAXJ - 1;
+    uint32_t eoff[W::MAXJ];
+    bool eok[W::MAXJ];
+#pragma unroll
+    for (int j = 0; j < W::MAXJ; ++j) {
+        const int e = (wave + D::NWAVE * j) * 64 + lane;
+        eok[j] = j < nj && e < W::NE;
+        const int r = e / D::RW, c = e - r * D::RW;
+        int gx = x0 + c - 3, gy = y0 + r - 3;
+        gx = gx < 0 ? gx + nx : (gx >= nx ? gx - nx : gx);
+        gy = gy < 0 ? gy + ny : (gy >= ny ? gy - ny : gy);
+        eoff[j] = eok[j] ? (uint32_t)(((size_t)gy * nx + gx) * 2) : 0u;
+    }
+    auto slot_of = [&](int z) { return (z - zb + 3) % W::NSLOT; };
+    auto issue = [&](int z, int slot) {
+        int zz = z % nz;
+        zz = zz < 0 ? zz + nz : zz;
+        const double *src = Q + (size_t)zz * plane * 2;
+        unsigned char *dst = ring + slot * W::SLOT;
+#pragma unroll
+        for (int j = 0; j < W::MAXJ; ++j) {
+            if (j < nj) {
+                const unsigned lds = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(
+                    (__attribute__((address_space(3))) unsigned char *)(dst + (wave + D::NWAVE * j) * 1024)));
+                const double *g = src + eoff[j];
+                if (eok[j])
+                    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                                 :: "s"(lds), "v"(g) : "memory", "m0");
+            }
+        }
+    };
+    // vmcnt: all loads, or all but the nj of the most recent plane
+    auto wait_planes = [&](bool keep_last) {
+        if (!keep_last) __builtin_amdgcn_s_waitcnt(0x0F70);
+        else if (nj == 2) __builtin_amdgcn_s_waitcnt(0x0F72);
+        else __builtin_amdgcn_s_waitcnt(0x0F71);
+    };
+    auto cell_at = [&](int slot, int r, int c) -> double2 {
+        return *reinterpret_cast<const double2 *>(ring + slot * W::SLOT + (r * D::RW + c) * 16);
+    };
+    auto u_at = [&](int slot, int r, int c) -> double {
+        return *reinterpret_cast<const double *>(ring + slot * W::SLOT + (r * D::RW + c) * 16);
+    };
+
+    for (int d = 0; d < W::NSLOT; ++d) issue(zb - 3 + d, d);
+    wait_planes(false);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    double2 qz[7];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) qz[d + 1] = cell_at(d, ty + 3, tx + 3);
+    qz[0] = qz[1];
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    issue(zb + 3, slot_of(zb + 3));
+    if (zb + 4 <= ze + 2) issue(zb + 4, slot_of(zb + 4));
+
+    const double lap = p.lapc;
+    const size_t col = (size_t)(y0 + ty) * nx + (x0 + tx);
+    bool nfbad = false;
+    for (int z = zb; z < ze; ++z) {
+        wait_planes(z + 4 <= ze + 2);
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (z + 5 <= ze + 2) issue(z + 5, slot_of(z + 5));
+        const int s0 = slot_of(z);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) qz[k] = qz[k + 1];
+        qz[6] = cell_at(slot_of(z + 3), ty + 3, tx + 3);
+        const double q0 = qz[3].x;
+        const double lx = stencil_L7(u_at(s0, ty + 3, tx), u_at(s0, ty + 3, tx + 1), u_at(s0, ty + 3, tx + 2), q0,
+                                     u_at(s0, ty + 3, tx + 4), u_at(s0, ty + 3, tx + 5), u_at(s0, ty + 3, tx + 6));
+        const double ly = stencil_L7(u_at(s0, ty, tx + 3), u_at(s0, ty + 1, tx + 3), u_at(s0, ty + 2, tx + 3), q0,
+                                     u_at(s0, ty + 4, tx + 3), u_at(s0, ty + 5, tx + 3), u_at(s0, ty + 6, tx + 3));
+        const double lz = stencil_L7(qz[0].x, qz[1].x, qz[2].x, q0, qz[4].x, qz[5].x, qz[6].x);
+        Cell<double, 2> r;
+        r.v[0] = qz[3].y;                 // du/dt = v
+        r.v[1] = lap * ((lx + ly) + lz);
+        if (active) {
+            Cell<double, 2> o;
+            if constexpr (EPI == EPI_RHS) {
+                o = r;
+            } else {
+                const double cc = p.c;
+                o.v[0] = qz[3].x + cc * r.v[0];
+                o.v[1] = qz[3].y + cc * r.v[1];
+            }
+            if (p.nf) finite_acc<double, 2>(nfbad, o);
+            __builtin_nontemporal_store(acx_dbl2{o.v[0], o.v[1]},
+                                        reinterpret_cast<acx_dbl2 *>(OUT + ((size_t)z * plane + col) * 2));
+        }
+    }
+    watch_flush(p.nf, nfbad);
+}
+
 } // namespace acx
