@@ -304,6 +304,163 @@ k_ctcs_z(const double *__restrict__ Pold, const double *__restrict__ P, double *
     }
 }
 
+// The same update on an LDS-DMA march (3D grids with an even x extent and
+// 16-byte aligned arrays): a block of 32 x 16 threads owns a 32 x 16 column
+// tile; each plane of P with a 2-cell x/y halo (36 x 20 cells) and the
+// tile's Pold and C go straight from memory into one slot of a 5-slot LDS
+// ring with global_load_lds_dwordx4 (16 bytes = two x-neighbours per lane).
+// At iteration z the slots hold planes z-1 .. z+3: plane z feeds the x/y
+// stencils and the update, z+1 and z+2 have landed (z+2 enters the column's
+// z-queue of planes z-2 .. z+2 in registers), z+3 and the plane z+4 issued in
+// this iteration are in flight.  Loads complete in issue order, so
+// s_waitcnt vmcnt(nj) (nj = this wavefront's loads of one plane) means plane
+// z+2 has landed in this wavefront's part; one s_barrier then publishes all
+// parts and frees the slot of plane z-1 for plane z+4.  The loads are issued
+// from inline assembly so that the compiler does not guard every LDS read
+// with a wait for all of them.  69.8 KB of LDS per block: two blocks per
+// CU.  The arithmetic is k_ctcs_z's term by term.
+struct CtDma {
+    static constexpr int BX = 32, BY = 16, R = 2;                 // tile, halo loaded (covers ST 1 and 2)
+    static constexpr int RW = BX + 2 * R, RH = BY + 2 * R;        // 36 x 20 cells of P
+    static constexpr int NP = RW / 2 * RH;                        // 360 entries (cell pairs) of P
+    static constexpr int NI = BX / 2 * BY;                        // 256 entries of Pold, and of C
+    static constexpr int NENT = NP + 2 * NI;                      // 872
+    static constexpr int SLOT = NENT * 16;                        // 13952 bytes
+    static constexpr int NSLOT = 5;
+    static constexpr int NWAVE = BX * BY / 64;                    // 8
+    static constexpr int NCHUNK = (NENT + 63) / 64;               // 14 wavefront loads per plane
+    static constexpr int MAXJ = (NCHUNK + NWAVE - 1) / NWAVE;     // 2
+};
+
+template <int ST>
+__global__ void __launch_bounds__(CtDma::BX *CtDma::BY) __attribute__((amdgpu_waves_per_eu(4)))
+k_ctcs_dma(const double *__restrict__ Pold, const double *__restrict__ P, double *__restrict__ Pnew,
+           const double *__restrict__ C, acx_ctcs c, int it, int zc, double *__restrict__ rcv)
+{
+    using D = CtDma;
+    __shared__ __attribute__((aligned(16))) unsigned char ring[D::NSLOT * D::SLOT];
+    const int nx = c.n[0], ny = c.n[1], nz = c.n[2];
+    const int ntx = (nx + D::BX - 1) / D::BX, nty = (ny + D::BY - 1) / D::BY;
+    const int b = xcd_logical_tile(blockIdx.x, gridDim.x);
+    const int x0 = (b % ntx) * D::BX, y0 = ((b / ntx) % nty) * D::BY, zb = (b / (ntx * nty)) * zc;
+    const int ze = min(zb + zc, nz);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * D::BX + tx;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int i = x0 + tx, j = y0 + ty;
+    const bool active = i < nx && j < ny;
+    const size_t plane = (size_t)nx * ny;
+    auto M = [](int a, int n) { a %= n; return a < 0 ? a + n : a; };
+
+    // this lane's entries: chunk q = wave + 8 k covers entries 64 q .. 64 q + 63
+    const int nj = wave + D::NWAVE * (D::MAXJ - 1) < D::NCHUNK ? D::MAXJ : D::MAXJ - 1;
+    const double *eptr[D::MAXJ];   // P, Pold or C at the entry's cell of plane 0
+    bool eok[D::MAXJ];
+#pragma unroll
+    for (int k = 0; k < D::MAXJ; ++k) {
+        const int e = (wave + D::NWAVE * k) * 64 + lane;
+        eok[k] = k < nj && e < D::NENT;
+        int gx, gy;
+        const double *base;
+        if (e < D::NP) {
+            const int r = e / (D::RW / 2), cp = e - r * (D::RW / 2);
+            gx = M(x0 - D::R + 2 * cp, nx);
+            gy = M(y0 - D::R + r, ny);
+            base = P;
+        } else {
+            const int e2 = (e - D::NP) % D::NI, r = e2 / (D::BX / 2), cp = e2 - r * (D::BX / 2);
+            gx = M(x0 + 2 * cp, nx);
+            gy = M(y0 + r, ny);
+            base = e < D::NP + D::NI ? Pold : C;
+        }
+        eptr[k] = base + (eok[k] ? (size_t)gy * nx + gx : 0);
+    }
+    auto slot_of = [&](int z) { return (z - zb + 2) % D::NSLOT; };
+    auto issue = [&](int z, int slot) {
+        const size_t zo = (size_t)M(z, nz) * plane;
+        unsigned char *dst = ring + slot * D::SLOT;
+#pragma unroll
+        for (int k = 0; k < D::MAXJ; ++k) {
+            if (k < nj) {
+                const unsigned lds = __builtin_amdgcn_readfirstlane((unsigned)(uintptr_t)(
+                    (__attribute__((address_space(3))) unsigned char *)(dst + (wave + D::NWAVE * k) * 1024)));
+                const double *g = eptr[k] + zo;
+                if (eok[k])
+                    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                                 :: "s"(lds), "v"(g) : "memory", "m0");
+            }
+        }
+    };
+    // vmcnt (gfx9 encoding, expcnt 7, lgkmcnt 15): all loads, or all but the
+    // nj of the most recent plane
+    auto wait_planes = [&](bool keep_last) {
+        if (!keep_last) __builtin_amdgcn_s_waitcnt(0x0F70);
+        else if (nj == 2) __builtin_amdgcn_s_waitcnt(0x0F72);
+        else __builtin_amdgcn_s_waitcnt(0x0F71);
+    };
+    // P at region row r, column cc of a slot; Pold / C of the tile cell (ty, tx)
+    auto pv = [&](int slot, int r, int cc) {
+        return *reinterpret_cast<const double *>(ring + slot * D::SLOT + (r * D::RW + cc) * 8);
+    };
+    auto tile_v = [&](int slot, int part) {
+        return *reinterpret_cast<const double *>(ring + slot * D::SLOT + D::NP * 16 + part * D::NI * 16 +
+                                                 (ty * D::BX + tx) * 8);
+    };
+
+    // prologue: planes zb-2 .. zb+2 into slots 0 .. 4, the column queue, then zb+3
+    for (int d = 0; d < D::NSLOT; ++d) issue(zb - 2 + d, d);
+    wait_planes(false);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    double q[5];
+#pragma unroll
+    for (int d = 0; d < 5; ++d) q[d] = pv(d, ty + D::R, tx + D::R);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (zb + 3 <= ze + 1) issue(zb + 3, slot_of(zb + 3));
+
+    const double invdx2 = 1.0 / (c.dx * c.dx), invdy2 = 1.0 / (c.dy * c.dy), invdz2 = 1.0 / (c.dz * c.dz);
+    const double dt = c.dt;
+    const size_t col = (size_t)M(j, ny) * nx + M(i, nx);
+    for (int z = zb; z < ze; ++z) {
+        if (z > zb) {
+            // plane z+2 landed: only plane z+3's loads may be outstanding
+            wait_planes(z + 3 <= ze + 1);
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int d = 0; d < 4; ++d) q[d] = q[d + 1];
+            q[4] = pv(slot_of(z + 2), ty + D::R, tx + D::R);
+        }
+        // every wavefront is past iteration z-1: its slot takes plane z+4
+        if (z + 4 <= ze + 1) issue(z + 4, slot_of(z + 4));
+        const int s0 = slot_of(z);
+        if (active) {
+            const double p0 = q[2];
+            const double po = tile_v(s0, 0), cc = tile_v(s0, 1);
+            const int r0 = ty + D::R, c0 = tx + D::R;
+            double pxx, pyy, pzz;
+            if (ST == 1) {
+                pxx = (pv(s0, r0, c0 + 1) - 2.0 * p0 + pv(s0, r0, c0 - 1)) * invdx2;
+                pyy = (pv(s0, r0 + 1, c0) - 2.0 * p0 + pv(s0, r0 - 1, c0)) * invdy2;
+                pzz = (q[3] - 2.0 * p0 + q[1]) * invdz2;
+            } else {
+                pxx = (-1.0 / 12 * pv(s0, r0, c0 + 2) + 4.0 / 3.0 * pv(s0, r0, c0 + 1) - 5.0 / 2.0 * p0 +
+                       4.0 / 3.0 * pv(s0, r0, c0 - 1) - 1.0 / 12.0 * pv(s0, r0, c0 - 2)) * invdx2;
+                pyy = (-1.0 / 12 * pv(s0, r0 + 2, c0) + 4.0 / 3.0 * pv(s0, r0 + 1, c0) - 5.0 / 2.0 * p0 +
+                       4.0 / 3.0 * pv(s0, r0 - 1, c0) - 1.0 / 12.0 * pv(s0, r0 - 2, c0)) * invdy2;
+                pzz = (-1.0 / 12 * q[4] + 4.0 / 3.0 * q[3] - 5.0 / 2.0 * p0 + 4.0 / 3.0 * q[1] -
+                       1.0 / 12.0 * q[0]) * invdz2;
+            }
+            double pn = 2.0 * p0 - po + cc * cc * dt * dt * (pxx + pyy + pzz);
+            if (i == c.src[0] && j == c.src[1] && z == c.src[2])
+                pn += ctcs_source(it * dt, c.f0) / (c.dx * c.dy * c.dz) * dt * dt;
+            if (rcv && i == c.rcv[0] && j == c.rcv[1] && z == c.rcv[2]) rcv[0] = p0;
+            Pnew[(size_t)z * plane + col] = pn;
+        }
+    }
+    // every load issued has been waited for by the last iteration (vmcnt 0)
+}
+
 // ---- host fields ---------------------------------------------------------
 
 static inline uint64_t splitmix64(uint64_t &s)
@@ -430,7 +587,19 @@ int acx_ctcs_update_f64(const acx_ctcs *c, const double *POLD, const double *P, 
         const int ra = hip_check(hipMallocAsync((void **)&rcv, sizeof(double), s), "ctcs receiver buffer");
         if (ra) return ra;
     }
-    if (c->n[0] >= 5 && c->n[1] >= 5 && c->n[2] >= 5) {
+    const bool aligned = ((((uintptr_t)POLD) | ((uintptr_t)P) | ((uintptr_t)C)) & 15) == 0;
+    if (c->n[0] >= 5 && c->n[1] >= 5 && c->n[2] >= 5 && c->n[0] % 2 == 0 && aligned &&
+        (size_t)c->n[0] * c->n[1] < 0xffffffffu) {
+        // 3D, LDS-DMA march: z-chunk from the round model, two blocks per CU
+        // (measured at 256^3: 144 -> 152 G cell-updates/s, bit-identical to
+        // the register-queue march below)
+        const int ntx = (c->n[0] + CtDma::BX - 1) / CtDma::BX, nty = (c->n[1] + CtDma::BY - 1) / CtDma::BY;
+        const int zc = auto_zc(ntx, nty, c->n[2], device_cus() * 2, 4);
+        const size_t nb = (size_t)ntx * nty * ((c->n[2] + zc - 1) / zc);
+        dim3 blk(CtDma::BX, CtDma::BY), grd((unsigned)nb);
+        if (c->stencil == 1) hipLaunchKernelGGL(k_ctcs_dma<1>, grd, blk, 0, s, POLD, P, PNEW, C, *c, it, zc, rcv);
+        else hipLaunchKernelGGL(k_ctcs_dma<2>, grd, blk, 0, s, POLD, P, PNEW, C, *c, it, zc, rcv);
+    } else if (c->n[0] >= 5 && c->n[1] >= 5 && c->n[2] >= 5) {
         // 3D: z-march; 32 planes per block, halved (down to 4) below 1024 blocks
         const size_t cols = (size_t)((c->n[0] + CT_BX - 1) / CT_BX) * ((c->n[1] + CT_BY - 1) / CT_BY);
         int zc = 32;
