@@ -223,6 +223,22 @@ enum { EPI_RHS = 0, EPI_AXPY = 1, EPI_COMB = 2, EPI_F32C = 3 };
 // One FP64 stage: OUT = a*U0 + b*Q + c*F(Q), epilogue chosen from (a, b).
 // watch: the stage's outputs feed the non-finite watch (a step watches only
 // its last stage: a NaN or Inf of an earlier stage reaches its output).
+// Small periodic one-plane ACOUSTIC4 grids with the shipped weights: a step
+// is one launch (k_step2d; up to 2^20 cells, where the four stage launches
+// are bound by launch latency rather than memory).
+static bool whole_step_2d(const acx_grid *g)
+{
+    return g->n[2] == 1 && g->system == ACX_ACOUSTIC4 && is_linear(g) && g->bc[0] == ACX_PERIODIC &&
+           g->bc[1] == ACX_PERIODIC && g->bc[2] == ACX_PERIODIC && (size_t)g->n[0] * g->n[1] <= (1u << 20);
+}
+
+static int step2d(const acx_grid *g, const double *IN, double *OUT, double dt, hipStream_t s)
+{
+    LinArgs la;
+    fill_lin(la, g);
+    return launch_step2d(la, IN, OUT, dt, s);
+}
+
 static int stage64(const acx_grid *g, const double *Q, const double *U0, double *OUT,
                    double a, double b, double c, int epi, hipStream_t s, bool watch = true)
 {
@@ -330,6 +346,11 @@ int acx_ssp_rks3_step_f64(const acx_grid *g, double *U, double *W0, double *W1, 
     if (!U || !W0 || !W1 || U == W0 || U == W1 || W0 == W1) return set_error(ACX_EINVAL, "U, W0, W1 must be distinct");
     if ((rc = nonfinite_entry("ssp_rks3_step_f64"))) return rc;
     hipStream_t s = (hipStream_t)stream;
+    if (whole_step_2d(g)) {
+        // the step in one launch into W0, copied back to U
+        if ((rc = step2d(g, U, W0, dt, s))) return rc;
+        return acx_vec_copy_f64(W0, U, (size_t)g->n[0] * g->n[1] * 4, s);
+    }
     // TSSSPRKS3, s = 4 (n = 2, r = 2): stage times 0, 1/2, 1, 1/2.
     if ((rc = stage64(g, U, nullptr, W0, 0.0, 1.0, dt / 2, EPI_AXPY, s, false))) return rc;
     if ((rc = stage64(g, W0, nullptr, W1, 0.0, 1.0, dt / 2, EPI_AXPY, s, false))) return rc;
@@ -381,8 +402,13 @@ static int step_graph_for(const acx_grid *g, double *U, double *W0, double *W1, 
     hipGraph_t graph = nullptr;
     rc = hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed), "step graph: begin capture");
     if (!rc) {
-        // the stage sequence of acx_ssp_rks3_step_f64, `steps` times
-        for (int k = 0; k < steps && !rc; ++k) {
+        // the stage sequence of acx_ssp_rks3_step_f64, `steps` times (a
+        // one-launch step alternates between U and W0, and ends in U)
+        const bool whole = whole_step_2d(g);
+        for (int k = 0; k < steps && !rc && whole; ++k)
+            rc = step2d(g, k % 2 ? W0 : U, k % 2 ? U : W0, dt, cs);
+        if (!rc && whole && steps % 2) rc = acx_vec_copy_f64(W0, U, (size_t)g->n[0] * g->n[1] * 4, cs);
+        for (int k = 0; k < steps && !rc && !whole; ++k) {
             rc = stage64(g, U, nullptr, W0, 0.0, 1.0, dt / 2, EPI_AXPY, cs, false);
             if (!rc) rc = stage64(g, W0, nullptr, W1, 0.0, 1.0, dt / 2, EPI_AXPY, cs, false);
             if (!rc)

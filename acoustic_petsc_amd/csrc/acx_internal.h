@@ -46,6 +46,9 @@ struct CwArgs {
 };
 
 int launch_linear(const LinArgs &a, int sys, int epi, bool fp32, hipStream_t s);
+// One whole SSP-RK3 (s = 4) step IN -> OUT of the linear ACOUSTIC4 operator on
+// a periodic one-plane grid (acx_step2d.hip); IN and OUT distinct.
+int launch_step2d(const LinArgs &a, const double *IN, double *OUT, double dt, hipStream_t s);
 int launch_cweno(const CwArgs &a, int sys, int epi, bool fp32, bool selective, hipStream_t s);
 // Tiles of the FP32 selective CWENO launch for a.zc: count, and a mask that is
 // 1 for every tile holding a flagged cell or a face neighbour of one.

@@ -255,9 +255,10 @@ def variant_2d(torch, acx, steps=200, N=256):
     per = 20
     tg, _ = time_steps(torch, lambda: acx.ssp_rks3_steps_graph(g, U, dt, per, work), steps // per, 1)
     return {"value": N * N * STAGES * steps / tg, "ms_per_step": 1e3 * tg / steps, "steps": steps,
-            "grid": [N, N, 1], "stepper": f"hipGraph replays of {per} captured steps (4 stage launches each)",
+            "grid": [N, N, 1], "stepper": f"hipGraph replays of {per} captured steps, one launch per step",
             "eager_value": N * N * STAGES * steps / t, "eager_ms_per_step": 1e3 * t / steps,
-            "note": "launch-bound: 65536 cells per stage launch"}
+            "note": "one launch per step (16 x 16 tiles with a 12-cell halo, the four stages in LDS); "
+                    "eager steps add a copy back into U"}
 
 
 def variant_wave2(torch, acx, N=256, steps=20):

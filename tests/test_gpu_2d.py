@@ -40,6 +40,40 @@ def oracle_rhs2d(orc, N, h, system, bc, U2, gamma=(1.0, 0.0)):
     return R, g2
 
 
+@pytest.mark.parametrize("N", [(12, 9), (16, 16), (70, 33)])
+def test_one_launch_2d_step_matches_the_oracle(N, acx, orc, torch_gpu):
+    """Periodic 2D linearised acoustics with the shipped weights steps in one
+    launch (tiles of 16 x 16 with a 12-cell halo, the four stages in LDS):
+    against the oracle's 2D RKS3 steps at 1e-12 on grids smaller than the
+    halo (wrapping several times), exactly one tile, and partial tiles; the
+    graph-replayed steps (alternating between U and W0, an odd count ending
+    with a copy) equal the eager ones bit for bit."""
+    torch = torch_gpu
+    h = 1.0 / N[0]
+    rng = np.random.default_rng(N[0] * 100 + N[1])
+    U2 = rng.standard_normal((N[1], N[0], 3))
+    U = np.zeros((1, N[1], N[0], 4))
+    U[0, ..., :3] = U2
+    grid = acx.Grid(n=(N[0], N[1], 1), h=h, system=1)
+    dt = acx.cfl_dt(grid, 0.4)
+    g2 = orc.G2((N[0], N[1]), h, 1, 0, 1.0, 0.0, 1.0, 1.0, (0, 0))
+    ref = U2.copy()
+    for _ in range(3):
+        orc.ts_solve("orc_rhs2d_cb", g2, ref, dt, dt, 0, 4)
+    print(f"[case] one-launch 2d step n={N}", flush=True)
+    a = torch.from_numpy(U).cuda()
+    b = a.clone()
+    wa = (torch.empty_like(a), torch.empty_like(a))
+    wb = (torch.empty_like(b), torch.empty_like(b))
+    for _ in range(3):
+        acx.ssp_rks3_step(grid, a, dt, wa)
+    acx.ssp_rks3_steps_graph(grid, b, dt, 3, wb)
+    got = a.cpu().numpy()
+    assert rel_l2(got[0, ..., :3], ref) <= TOL
+    assert np.all(got[..., 3] == 0.0)
+    assert torch.equal(a, b)
+
+
 @pytest.mark.parametrize("system,bc", [(0, (0, 0)), (0, (1, 1)), (0, (1, 0)), (1, (0, 0))],
                          ids=["wave2-periodic", "wave2-adiabatic", "wave2-mixed", "acoustic-periodic"])
 def test_rhs_and_steps_match_2d_oracle(system, bc, acx, orc, torch_gpu):
