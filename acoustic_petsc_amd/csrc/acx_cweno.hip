@@ -226,7 +226,8 @@ k_cweno(CwArgs p)
         const size_t cell = (size_t)(m - 1) * plane + ocol;
         Cell<T, NV> e0, e1, e2;
         if (fin) {
-            if constexpr (EPI == EPI_COMB) e0 = load_cell<T, NV>(U0, cell);
+            // COMB: U0 is loaded at the epilogue instead (8 fewer live VGPRs
+            // through the weights, where the registers spill)
             if constexpr (EPI == EPI_F32C) {
                 if (has_d) e0 = load_cell<T, NV>(D, cell);
                 if (p.final_stage) {
@@ -366,6 +367,7 @@ k_cweno(CwArgs p)
 #pragma unroll
                         for (int v = 0; v < NV; ++v) o.v[v] = q.v[v] + T(p.c) * r.v[v];
                     } else {
+                        e0 = load_cell<T, NV>(U0, cell);
 #pragma unroll
                         for (int v = 0; v < NV; ++v) o.v[v] = T(p.a) * e0.v[v] + T(p.b) * q.v[v] + T(p.c) * r.v[v];
                     }
