@@ -76,7 +76,10 @@ int acx_stage_f64(const acx_grid *g, const double *Q, const double *U0, double *
                   double a, double b, double c, void *stream);
 
 /* One step of TSSSPRKS3 with 4 stages (hype.c:181-182): U <- step(U, dt).
- * W0, W1 are device work vectors of the same size as U. */
+ * W0, W1 are device work vectors of the same size as U; their contents after
+ * the call are unspecified.  Four fused stage launches; on periodic one-plane
+ * ACOUSTIC4 grids with the shipped weights and up to 2^20 cells, one launch
+ * of the whole step into W0 and a copy back into U. */
 int acx_ssp_rks3_step_f64(const acx_grid *g, double *U, double *W0, double *W1, double dt, void *stream);
 
 /* nsteps steps of acx_ssp_rks3_step_f64 replayed from a hipGraph of their
