@@ -66,8 +66,12 @@ def test_reference_program_trace(acx, orc, torch_gpu):
         assert abs(v - float(t)) <= half * (1 + 1e-9) + 1e-12 * scale, (i, v, t)
 
 
+# even x extents take the LDS-DMA march (including a grid smaller than its
+# halo and one with partial tiles on every axis), odd ones the register march
 @pytest.mark.parametrize("n,d,stencil", [((20, 18, 16), (0.5, 0.4, 0.3), 1),
                                          ((20, 18, 16), (0.5, 0.4, 0.3), 2),
+                                         ((6, 5, 5), (0.3, 0.3, 0.3), 2),
+                                         ((34, 20, 21), (0.2, 0.3, 0.25), 2),
                                          ((7, 33, 9), (0.2, 0.25, 0.2), 2)])
 def test_fields_match_oracle(acx, orc, torch_gpu, n, d, stencil):
     rng = np.random.default_rng(7 + stencil)
