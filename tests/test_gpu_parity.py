@@ -66,6 +66,11 @@ CASES = [
     ("ac4-lin-tiny", (4, 5, 6), 1, (1.0, 0.0), (0, 2, 3), "rough"),
     ("ac4-nl-tiny", (5, 4, 4), 1, (0.85, 0.85), (0, 0, 2), "rough"),
     ("w2-nl-tiny", (4, 4, 5), 0, (0.85, 0.85), (1, 0, 1), "rough"),
+    # all-periodic linear grids take the LDS-DMA marches: smaller than the
+    # halo (every plane and row loaded several times) and ragged on every axis
+    ("ac4-lin-tiny-per", (4, 5, 4), 1, (1.0, 0.0), (0, 0, 0), "rough"),
+    ("w2-lin-tiny-per", (5, 4, 4), 0, (1.0, 0.0), (0, 0, 0), "rough"),
+    ("w2-lin-per-ragged", (70, 33, 21), 0, (1.0, 0.0), (0, 0, 0), "smooth"),
 ]
 
 
