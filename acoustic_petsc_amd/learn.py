@@ -304,6 +304,7 @@ class SelectiveStepper:
         self.redone_windows = 0
         self.judged_windows = 0
         self._saved = None       # (hi, lo) at the start of the current window
+        self._swap_in = None     # the free (hi, lo) pair that replaces the saved one after a window's first step
         self._window_dts = []
 
     def _mlp_flags(self, stream=None):
@@ -328,6 +329,7 @@ class SelectiveStepper:
     def refresh_flags(self, stream=None):
         """Judge the finished window (rerun it with full WENO on a miss), then
         take new flags from the MLP and the current weights."""
+        import torch
         have_window = self._saved is not None and self._window_dts
         if self.use_judge and have_window:
             self._mlp_flags(stream)
@@ -356,17 +358,24 @@ class SelectiveStepper:
         # the patch tiles of these flags, kept until the next refresh
         self.mask = sel_mask(self.grid, self.flags, self.mask, stream)
         if self.use_judge:
-            if self._saved is None:
-                self._saved = (self.state.hi.clone(), self.state.lo.clone())
-            else:
-                self._saved[0].copy_(self.state.hi)
-                self._saved[1].copy_(self.state.lo)
+            # The window's start state is kept without a copy: the pair that
+            # holds it now stays out of the state's ping-pong until the next
+            # refresh (the first step reads it, and its place as the next
+            # output is taken by the free pair); three pairs in all.
+            old = self._saved
+            self._saved = (self.state.hi, self.state.lo)
+            self._swap_in = old if old is not None else (torch.empty_like(self.state.hi),
+                                                        torch.empty_like(self.state.lo))
         self._window_dts = []
 
     def step(self, dt):
         if self.nsteps % self.refresh == 0:
             self.refresh_flags()
         self.state.step(dt, flags=self.flags, mask=self.mask)
+        if self._swap_in is not None:
+            # the step wrote the free pair's partner; the saved pair would be the next output
+            self.state._hi2, self.state._lo2 = self._swap_in
+            self._swap_in = None
         self._window_dts.append(dt)
         self.nsteps += 1
 
