@@ -366,8 +366,13 @@ __global__ void __launch_bounds__(64 * IND_WAVES) k_indicator(const MlpPacked *_
 #pragma unroll
                 for (int q = 0; q < ACX_MLP_NFEAT; ++q) f[q] = 0.f;
             }
+            // four 16-byte stores per row (a 2-byte store per feature put four
+            // lanes on each LDS bank: rows are 80 bytes apart)
+            bf16x8 c[ACX_MLP_IN / 8];
 #pragma unroll
-            for (int q = 0; q < ACX_MLP_IN; ++q) F[lane][q] = (__bf16)(q < ACX_MLP_NFEAT ? f[q] : 0.f);
+            for (int q = 0; q < ACX_MLP_IN; ++q) c[q / 8][q % 8] = (__bf16)(q < ACX_MLP_NFEAT ? f[q] : 0.f);
+#pragma unroll
+            for (int h = 0; h < ACX_MLP_IN / 8; ++h) *reinterpret_cast<bf16x8 *>(&F[lane][8 * h]) = c[h];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
