@@ -26,7 +26,7 @@ struct Step2d {
     static constexpr int H = 12;                   // halo: 4 stages x 3 cells
     static constexpr int R0 = T + 2 * H;           // 40: state region
     static constexpr int R1 = R0 - 6, R2 = R1 - 6, R3 = R2 - 6;   // 34, 28, 22 (stage outputs 1-3)
-    static constexpr int NT = 512;                 // threads per block
+    static constexpr int NT = 1024;                // threads per block
     // LDS: buffer A holds the state (R0^2 cells), later stage 2 (R2^2);
     // buffer B stage 1 (R1^2), later stage 3 (R3^2).  Each as two arrays of
     // 16-byte halves, (p, u) and (v, w), so that a wavefront reading
