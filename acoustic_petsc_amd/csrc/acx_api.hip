@@ -533,11 +533,15 @@ int acx_nonfinite_clear(void)
 namespace acx {
 
 // Patch tiling: 30 x 14 x zc cells, zc read once per process, so a tile mask
-// made by acx_sel_mask stays valid for every later selective step.
+// made by acx_sel_mask stays valid for every later selective step.  Measured
+// on a steep pulse (flags of 1.2-1.9 % of the (cell, component) pairs) at
+// 256^3 and 384^3: zc = 8 leaves about one round of resident patch blocks
+// (508 tiles at 384^3) and took the step 3.32 -> 3.22 ms at 384^3 against
+// zc = 4 (968 tiles, two rounds); zc = 2 and 16 were slower (3.52, 3.48 ms).
 static void fill_cw_patch(CwArgs &a, const acx_grid *g)
 {
     fill_cw(a, g);
-    static const int zc = env_int("ACX_ZC_PATCH", 4);
+    static const int zc = env_int("ACX_ZC_PATCH", 8);
     a.zc = zc < 1 ? 1 : (zc > g->n[2] ? g->n[2] : zc);
 }
 
@@ -559,7 +563,7 @@ static int stage32c_sel(const acx_grid *g, const float *HI, const float *D, cons
     ca.q = HI; ca.d = D; ca.lo = LO; ca.out = OUT; ca.out2 = OUT2;
     ca.b = b; ca.c = c; ca.final_stage = final_stage;
     ca.flags = flags;
-    ca.tilemask = mask;
+    ca.tlist = reinterpret_cast<const int *>(mask + sel_list_offset(cweno_sel_tiles(ca)));
     if (!final_stage) ca.nf = nullptr;
     return launch_cweno(ca, g->system, EPI_F32C, true, true, s);
 }
@@ -608,7 +612,9 @@ extern "C" int acx_ssp_rks3_step_f32c_sel(const acx_grid *g, const float *HI, co
     fill_cw_patch(ca, g);
     // the tile mask of this call, from the stream-ordered pool of s
     uint8_t *mask = nullptr;
-    if ((rc = hip_check(hipMallocAsync((void **)&mask, cweno_sel_tiles(ca), s), "selective step: tile mask"))) return rc;
+    if ((rc = hip_check(hipMallocAsync((void **)&mask, sel_buffer_bytes(cweno_sel_tiles(ca)), s),
+                        "selective step: tile mask")))
+        return rc;
     rc = launch_cweno_tilemask(ca, flags, mask, s);
     if (!rc) rc = sel_stages(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags, mask, dt, s);
     const int rf = hip_check(hipFreeAsync(mask, s), "selective step: tile mask free");
@@ -620,7 +626,7 @@ extern "C" size_t acx_sel_mask_bytes(const acx_grid *g)
     if (check_grid(g)) return 0;
     CwArgs ca;
     fill_cw_patch(ca, g);
-    return cweno_sel_tiles(ca);
+    return sel_buffer_bytes(cweno_sel_tiles(ca));
 }
 
 extern "C" int acx_sel_mask(const acx_grid *g, const uint8_t *flags, uint8_t *mask, void *stream)
@@ -628,6 +634,7 @@ extern "C" int acx_sel_mask(const acx_grid *g, const uint8_t *flags, uint8_t *ma
     int rc = check_grid(g);
     if (rc) return rc;
     if (!flags || !mask) return set_error(ACX_EINVAL, "selective mask: null pointer");
+    if ((uintptr_t)mask % 4) return set_error(ACX_EINVAL, "selective mask: buffer not 4-byte aligned");
     CwArgs ca;
     fill_cw_patch(ca, g);
     return launch_cweno_tilemask(ca, flags, mask, (hipStream_t)stream);
@@ -640,6 +647,7 @@ extern "C" int acx_ssp_rks3_step_f32c_selm(const acx_grid *g, const float *HI, c
     int rc = check_sel(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags);
     if (rc) return rc;
     if (!mask) return set_error(ACX_EINVAL, "selective step: mask is NULL");
+    if ((uintptr_t)mask % 4) return set_error(ACX_EINVAL, "selective step: mask not 4-byte aligned");
     hipStream_t s = (hipStream_t)stream;
     if (env_int("ACX_SEL_FULL", 0)) return sel_stages(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags, nullptr, dt, s);
     return sel_stages(g, HI, LO, HI_OUT, LO_OUT, D0, D1, flags, mask, dt, s);
@@ -651,7 +659,7 @@ extern "C" int acx_ssp_rks3_step_f32c_selm(const acx_grid *g, const float *HI, c
 // holds its own copy.  Host code never passes the array's name to a kernel
 // (on the host it is only a stand-in symbol): the device address is looked up
 // once per device with hipGetSymbolAddress and kept in g_scratch_ptr.
-static __device__ __align__(256) uint8_t scratch[ACX_SEL_NOALLOC_MAX_TILES];
+static __device__ __align__(256) uint8_t scratch[acx::sel_buffer_bytes(ACX_SEL_NOALLOC_MAX_TILES)];
 static uint8_t *g_scratch_ptr[64];
 static std::mutex g_scratch_mu;
 

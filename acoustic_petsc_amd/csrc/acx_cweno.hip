@@ -25,6 +25,8 @@
 // inside a 32-lane row, y-neighbours through LDS, z-neighbours stay in
 // registers.  Selective mode (per-cell flags) replaces the full CWENO by
 // the linear 4th-order polynomial wherever the flag bit of a component is 0.
+#include <algorithm>
+
 #include "acx_device.h"
 #include "acx_internal.h"
 #include "acx_cweno_core.h"
@@ -114,290 +116,299 @@ k_cweno(CwArgs p)
     T *__restrict__ OUT2 = static_cast<T *>(p.out2);
 
     const int nx = p.nx, ny = p.ny, nz = p.nz;
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
-    // Selective patch: the few marked tiles cluster in space (along a front),
-    // so they are spread over the XCDs round-robin (block b -> tile b, XCD
-    // b mod 8) instead of giving each XCD a contiguous range of tiles, which
-    // piled the marked tiles onto a few XCDs.
-    const int tile = SEL ? (int)blockIdx.x : xcd_logical_tile(blockIdx.x, gridDim.x);
-    if constexpr (SEL) {
-        if (p.tilemask && !p.tilemask[tile]) return;   // whole block: the linear pass holds this tile
-    }
-    const int gx0 = (tile % p.ntx) * (BX - 2) - 1, gy0 = ((tile / p.ntx) % p.nty) * (BY - 2) - 1;
-    const int gx = gx0 + tx, gy = gy0 + ty;
-    const int zb = (tile / (p.ntx * p.nty)) * p.zc, ze = min(zb + p.zc, nz);
-    const bool interior = tx >= 1 && tx <= BX - 2 && ty >= 1 && ty <= BY - 2 && gx < nx && gy < ny;
+    // Selective patch: the list of marked tiles (count, then tile indices in
+    // ascending order), or every tile without a list; block b takes entries
+    // b, b + grid, ...  The marked tiles cluster in space (along a front), and
+    // consecutive entries go to consecutive blocks, i.e. round-robin over the
+    // XCDs.  Only marked tiles are visited: a grid of one block per tile that
+    // returned at once on an unmarked one took longer for the empty blocks
+    // than for the marked.  Otherwise one tile per block, XCD-contiguous.
+    int item = blockIdx.x;
+    const int nitems = SEL ? (p.tlist ? p.tlist[0] : p.ntx * p.nty * p.ntz) : 1;
+    if (SEL && item >= nitems) return;
     bool nfbad = false;   // non-finite watch of this thread's outputs
-    const size_t plane = (size_t)nx * ny;
-    const int kx = WALLS ? p.bc[0] : 0, ky = WALLS ? p.bc[1] : 0, kz = WALLS ? p.bc[2] : 0;   // boundary kinds
+    do {
+        const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * BX + tx;
+        const int tile = SEL ? (p.tlist ? p.tlist[1 + item] : item) : xcd_logical_tile(blockIdx.x, gridDim.x);
+        const int gx0 = (tile % p.ntx) * (BX - 2) - 1, gy0 = ((tile / p.ntx) % p.nty) * (BY - 2) - 1;
+        const int gx = gx0 + tx, gy = gy0 + ty;
+        const int zb = (tile / (p.ntx * p.nty)) * p.zc, ze = min(zb + p.zc, nz);
+        const bool interior = tx >= 1 && tx <= BX - 2 && ty >= 1 && ty <= BY - 2 && gx < nx && gy < ny;
+        const size_t plane = (size_t)nx * ny;
+        const int kx = WALLS ? p.bc[0] : 0, ky = WALLS ? p.bc[1] : 0, kz = WALLS ? p.bc[2] : 0;   // boundary kinds
 
-    // own column (mapped) and its ghost sign
-    float sx0 = 1.f, sy0 = 1.f;
-    const uint32_t ocol = (uint32_t)(map_axis(gy, ny, ky, sy0) * nx + map_axis(gx, nx, kx, sx0));
+        // own column (mapped) and its ghost sign
+        float sx0 = 1.f, sy0 = 1.f;
+        const uint32_t ocol = (uint32_t)(map_axis(gy, ny, ky, sy0) * nx + map_axis(gx, nx, kx, sx0));
 
-    // ring cells this thread loads (at most 2 of the (BX+4)(BY+4) region)
-    constexpr int NREG = LX * LY;
-    constexpr int PER = (NREG + NT - 1) / NT;
-    uint32_t rcol[PER];
-    float rsx[PER], rsy[PER];
-#pragma unroll
-    for (int r = 0; r < PER; ++r) {
-        const int idx = tid + r * NT;
-        rcol[r] = 0;
-        rsx[r] = rsy[r] = 1.f;
-        if (idx < NREG) {
-            const int lx = idx % LX, ly = idx / LX;
-            rcol[r] = (uint32_t)(map_axis(gy0 - 2 + ly, ny, ky, rsy[r]) * nx + map_axis(gx0 - 2 + lx, nx, kx, rsx[r]));
-        }
-    }
-    const bool has_d = (EPI == EPI_F32C) && D != nullptr;
-    auto zoff = [&](int z, float &s) -> size_t {
-        s = 1.f;
-        return (size_t)map_axis(z, nz, kz, s) * plane;
-    };
-    auto slot = [](int m) { return ((m % L::NR) + L::NR) % L::NR; };
-    // Plane loads keep the raw words; the FP32 increment and the ghost sign
-    // are applied when the plane is stored, so no wait sits at the load.
-    struct Raw {
-        Cell<T, NV> q, d;
-        float s;
-    };
-    auto load_plane = [&](int z, Raw(&buf)[PER]) {
-        float s;
-        const size_t off = zoff(z, s);
-#pragma unroll
-        for (int r = 0; r < PER; ++r)
-            if (tid + r * NT < NREG) {
-                buf[r].q = load_cell<T, NV>(Q, off + rcol[r]);
-                if (has_d) buf[r].d = load_cell<T, NV>(D, off + rcol[r]);
-                buf[r].s = s;
-            }
-    };
-    auto store_plane = [&](int z, const Raw(&buf)[PER]) {
-        const int sl = slot(z);
+        // ring cells this thread loads (at most 2 of the (BX+4)(BY+4) region)
+        constexpr int NREG = LX * LY;
+        constexpr int PER = (NREG + NT - 1) / NT;
+        uint32_t rcol[PER];
+        float rsx[PER], rsy[PER];
 #pragma unroll
         for (int r = 0; r < PER; ++r) {
             const int idx = tid + r * NT;
+            rcol[r] = 0;
+            rsx[r] = rsy[r] = 1.f;
             if (idx < NREG) {
-                Cell<T, NV> c = buf[r].q;
-                if (has_d) {
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) c.v[v] += buf[r].d.v[v];
-                }
-                if constexpr (WALLS) apply_bc_sign<T, NV>(c, rsx[r], rsy[r], buf[r].s);
-#pragma unroll
-                for (int v = 0; v < NS; ++v) ring[sl][v][idx / LX][idx % LX] = c.v[v];
+                const int lx = idx % LX, ly = idx / LX;
+                rcol[r] = (uint32_t)(map_axis(gy0 - 2 + ly, ny, ky, rsy[r]) * nx + map_axis(gx0 - 2 + lx, nx, kx, rsx[r]));
             }
         }
-    };
-
-    // prologue: planes zb-3 .. zb+1
-    for (int z = zb - 3; z <= zb + 1; ++z) {
-        Raw b0[PER];
-        load_plane(z, b0);
-        store_plane(z, b0);
-    }
-    __syncthreads();
-
-    const T K = T(p.K), irho = T(p.invrho), sL = T(p.s);
-    T zp_prev[NS], Fz_prev[NS], dxy_prev[NS];
+        const bool has_d = (EPI == EPI_F32C) && D != nullptr;
+        auto zoff = [&](int z, float &s) -> size_t {
+            s = 1.f;
+            return (size_t)map_axis(z, nz, kz, s) * plane;
+        };
+        auto slot = [](int m) { return ((m % L::NR) + L::NR) % L::NR; };
+        // Plane loads keep the raw words; the FP32 increment and the ghost sign
+        // are applied when the plane is stored, so no wait sits at the load.
+        struct Raw {
+            Cell<T, NV> q, d;
+            float s;
+        };
+        auto load_plane = [&](int z, Raw(&buf)[PER]) {
+            float s;
+            const size_t off = zoff(z, s);
 #pragma unroll
-    for (int v = 0; v < NS; ++v) { zp_prev[v] = T(0); Fz_prev[v] = T(0); dxy_prev[v] = T(0); }
-    T vcen_prev = T(0);
-    const int lx = tx + 2, ly = ty + 2;
+            for (int r = 0; r < PER; ++r)
+                if (tid + r * NT < NREG) {
+                    buf[r].q = load_cell<T, NV>(Q, off + rcol[r]);
+                    if (has_d) buf[r].d = load_cell<T, NV>(D, off + rcol[r]);
+                    buf[r].s = s;
+                }
+        };
+        auto store_plane = [&](int z, const Raw(&buf)[PER]) {
+            const int sl = slot(z);
+#pragma unroll
+            for (int r = 0; r < PER; ++r) {
+                const int idx = tid + r * NT;
+                if (idx < NREG) {
+                    Cell<T, NV> c = buf[r].q;
+                    if (has_d) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) c.v[v] += buf[r].d.v[v];
+                    }
+                    if constexpr (WALLS) apply_bc_sign<T, NV>(c, rsx[r], rsy[r], buf[r].s);
+#pragma unroll
+                    for (int v = 0; v < NS; ++v) ring[sl][v][idx / LX][idx % LX] = c.v[v];
+                }
+            }
+        };
 
-    uint8_t fl_next = 0xff;
-    if constexpr (SEL) {
-        float s;
-        fl_next = p.flags[zoff(zb - 1, s) + ocol];
-    }
-    for (int m = zb - 1; m <= ze; ++m) {
-        // prefetch plane m+3 of the region, the flags of plane m+1 and the
-        // epilogue operands of plane m-1
-        Raw nxt[PER];
-        const bool pf = m + 3 <= ze + 2;
-        if (pf) load_plane(m + 3, nxt);
-        const uint8_t fl = fl_next;
+        // prologue: planes zb-3 .. zb+1
+        for (int z = zb - 3; z <= zb + 1; ++z) {
+            Raw b0[PER];
+            load_plane(z, b0);
+            store_plane(z, b0);
+        }
+        __syncthreads();
+
+        const T K = T(p.K), irho = T(p.invrho), sL = T(p.s);
+        T zp_prev[NS], Fz_prev[NS], dxy_prev[NS];
+#pragma unroll
+        for (int v = 0; v < NS; ++v) { zp_prev[v] = T(0); Fz_prev[v] = T(0); dxy_prev[v] = T(0); }
+        T vcen_prev = T(0);
+        const int lx = tx + 2, ly = ty + 2;
+
+        uint8_t fl_next = 0xff;
         if constexpr (SEL) {
             float s;
-            if (m + 1 <= ze) fl_next = p.flags[zoff(m + 1, s) + ocol];
+            fl_next = p.flags[zoff(zb - 1, s) + ocol];
         }
-        const bool fin = m >= zb + 1 && interior;
-        const size_t cell = (size_t)(m - 1) * plane + ocol;
-        Cell<T, NV> e0, e1, e2;
-        if (fin) {
-            // COMB: U0 is loaded at the epilogue instead (8 fewer live VGPRs
-            // through the weights, where the registers spill)
-            if constexpr (EPI == EPI_F32C) {
-                if (has_d) e0 = load_cell<T, NV>(D, cell);
-                if (p.final_stage) {
-                    e1 = load_cell<T, NV>(LO, cell);
-                    e2 = load_cell<T, NV>(Q, cell);
-                }
+        for (int m = zb - 1; m <= ze; ++m) {
+            // prefetch plane m+3 of the region, the flags of plane m+1 and the
+            // epilogue operands of plane m-1
+            Raw nxt[PER];
+            const bool pf = m + 3 <= ze + 2;
+            if (pf) load_plane(m + 3, nxt);
+            const uint8_t fl = fl_next;
+            if constexpr (SEL) {
+                float s;
+                if (m + 1 <= ze) fl_next = p.flags[zoff(m + 1, s) + ocol];
             }
-        }
-        T vcen = T(0);
-        if constexpr (SYS == ACX_WAVE2) {
-            float s;
-            const size_t idx = (zoff(m, s) + ocol) * NV + 1;
-            vcen = Q[idx];
-            if constexpr (EPI == EPI_F32C) { if (D) vcen += D[idx]; }
-        }
-
-        const int sm2 = slot(m - 2), s0 = slot(m - 1), s1 = slot(m), s2 = slot(m + 1), sp2 = slot(m + 2);
-        T fpx[NS], fmx[NS], fpy[NS], fmz[NS], fpz[NS];
-#pragma unroll
-        for (int v = 0; v < NS; ++v) {
-            const T(*P0)[LX] = ring[s0][v];
-            const T(*P1)[LX] = ring[s1][v];
-            const T(*P2)[LX] = ring[s2][v];
-            const T(*M2)[LX] = ring[sm2][v];
-            const T(*Q2)[LX] = ring[sp2][v];
-            // stencil slot q -> LDS (slot order of cweno_weights)
-            auto get = [&](int q) -> T {
-                switch (q) {
-                case 0: return P1[ly][lx];
-                case 1: return P1[ly][lx - 2];
-                case 2: return P1[ly][lx - 1];
-                case 3: return P1[ly][lx + 1];
-                case 4: return P1[ly][lx + 2];
-                case 5: return P1[ly - 2][lx];
-                case 6: return P1[ly - 1][lx];
-                case 7: return P1[ly + 1][lx];
-                case 8: return P1[ly + 2][lx];
-                case 9: return M2[ly][lx];
-                case 10: return P0[ly][lx];
-                case 11: return P2[ly][lx];
-                case 12: return Q2[ly][lx];
-                case 13: return P1[ly + 1][lx + 1];   // xy: ++ +- -+ --
-                case 14: return P1[ly - 1][lx + 1];
-                case 15: return P1[ly + 1][lx - 1];
-                case 16: return P1[ly - 1][lx - 1];
-                case 17: return P2[ly + 1][lx];       // yz (y,z): ++ +- -+ --
-                case 18: return P0[ly + 1][lx];
-                case 19: return P2[ly - 1][lx];
-                case 20: return P0[ly - 1][lx];
-                case 21: return P2[ly][lx + 1];       // zx (z,x): ++ +- -+ --
-                case 22: return P2[ly][lx - 1];
-                case 23: return P0[ly][lx + 1];
-                case 24: return P0[ly][lx - 1];
-                case 25: return P2[ly + 1][lx + 1];   // corners (x,y,z): +++ -++ +-+ ++- --+ -+- +-- ---
-                case 26: return P2[ly + 1][lx - 1];
-                case 27: return P2[ly - 1][lx + 1];
-                case 28: return P0[ly + 1][lx + 1];
-                case 29: return P2[ly - 1][lx - 1];
-                case 30: return P0[ly + 1][lx - 1];
-                case 31: return P0[ly - 1][lx + 1];
-                default: return P0[ly - 1][lx - 1];
-                }
-            };
-
-            Axis3<T> ax;
-            bool full = true;
-            if constexpr (SEL) full = (fl >> v) & 1;
-            if (full) {
-                cweno_axes_lds<T>(get, p, ax);
-            } else {
-                axis_linear(get(1), get(2), get(0), get(3), get(4), ax.c1[0], ax.c2[0], ax.c3[0]);
-                axis_linear(get(5), get(6), get(0), get(7), get(8), ax.c1[1], ax.c2[1], ax.c3[1]);
-                axis_linear(get(9), get(10), get(0), get(11), get(12), ax.c1[2], ax.c2[2], ax.c3[2]);
-            }
-            T m3[3], p3[3];
-            face_q<T, SYS>(get(0), ax, m3, p3);
-            fmx[v] = m3[0]; fpx[v] = p3[0];
-            s_fym[v][ty][tx] = m3[1];
-            fpy[v] = p3[1];
-            fmz[v] = m3[2]; fpz[v] = p3[2];
-            asm volatile("" ::: "memory");   // one component's stencil live at a time
-        }
-        __syncthreads();   // B1: -y states visible; every read of plane m-1 is done
-
-        if (pf) store_plane(m + 3, nxt);   // into the slot of plane m-2
-
-        T Fx[NS], Fy[NS];
-        {
-            T Lf[NS], Rf[NS];
-#pragma unroll
-            for (int v = 0; v < NS; ++v) { Lf[v] = fpx[v]; Rf[v] = __shfl_down(fmx[v], 1, 32); }
-            face_flux<T, SYS, NS>(0, Lf, Rf, K, irho, sL, Fx);
-#pragma unroll
-            for (int v = 0; v < NS; ++v) { Lf[v] = fpy[v]; Rf[v] = ty + 1 < BY ? s_fym[v][ty + 1][tx] : T(0); }
-            face_flux<T, SYS, NS>(1, Lf, Rf, K, irho, sL, Fy);
-        }
-#pragma unroll
-        for (int v = 0; v < NS; ++v) s_Fy[v][ty][tx] = Fy[v];
-        __syncthreads();   // B2: fluxes and plane m+2 visible
-
-        T dxy[NS];
-#pragma unroll
-        for (int v = 0; v < NS; ++v) {
-            const T Fxm = __shfl_up(Fx[v], 1, 32);
-            const T Fym = ty > 0 ? s_Fy[v][ty - 1][tx] : T(0);
-            dxy[v] = (Fx[v] - Fxm) + (Fy[v] - Fym);
-        }
-
-        if (m >= zb) {
-            T Fz[NS], Lf[NS], Rf[NS];
-#pragma unroll
-            for (int v = 0; v < NS; ++v) { Lf[v] = zp_prev[v]; Rf[v] = fmz[v]; }
-            face_flux<T, SYS, NS>(2, Lf, Rf, K, irho, sL, Fz);
+            const bool fin = m >= zb + 1 && interior;
+            const size_t cell = (size_t)(m - 1) * plane + ocol;
+            Cell<T, NV> e0, e1, e2;
             if (fin) {
-                Cell<T, NV> r;
-                if constexpr (SYS == ACX_ACOUSTIC4) {
-                    const T ih = T(p.inv_h);
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) r.v[v] = -ih * (dxy_prev[v] + (Fz[v] - Fz_prev[v]));
-                } else {
-                    r.v[0] = vcen_prev;
-                    r.v[1] = T(p.c2) * (dxy_prev[0] + (Fz[0] - Fz_prev[0]));
-                }
-                Cell<T, NV> o;
-                if constexpr (EPI == EPI_RHS) {
-                    o = r;
-                } else if constexpr (EPI == EPI_AXPY || EPI == EPI_COMB) {
-                    // Q of the own (in-domain) cell of plane m-1 is still in the ring
-                    Cell<T, NV> q;
-                    if constexpr (NS == NV) {
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) q.v[v] = ring[s0][v][ly][lx];
-                    } else {
-                        q = load_cell<T, NV>(Q, cell);
-                    }
-                    if constexpr (EPI == EPI_AXPY) {
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) o.v[v] = q.v[v] + T(p.c) * r.v[v];
-                    } else {
-                        e0 = load_cell<T, NV>(U0, cell);
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) o.v[v] = T(p.a) * e0.v[v] + T(p.b) * q.v[v] + T(p.c) * r.v[v];
-                    }
-                } else {
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) o.v[v] = (has_d ? T(p.b) * e0.v[v] : T(0)) + T(p.c) * r.v[v];
+                // COMB: U0 is loaded at the epilogue instead (8 fewer live VGPRs
+                // through the weights, where the registers spill)
+                if constexpr (EPI == EPI_F32C) {
+                    if (has_d) e0 = load_cell<T, NV>(D, cell);
                     if (p.final_stage) {
-                        Cell<T, NV> nh, nl;
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) {
-                            float t = (float)e1.v[v] + (float)o.v[v];
-                            float sm, er;
-                            two_sum((float)e2.v[v], t, sm, er);
-                            nh.v[v] = sm;
-                            nl.v[v] = er;
-                        }
-                        store_cell<T, NV>(OUT2, cell, nl);
-                        o = nh;
+                        e1 = load_cell<T, NV>(LO, cell);
+                        e2 = load_cell<T, NV>(Q, cell);
                     }
                 }
-                if (p.nf) finite_acc<T, NV>(nfbad, o);
-                store_cell<T, NV>(OUT, cell, o);
+            }
+            T vcen = T(0);
+            if constexpr (SYS == ACX_WAVE2) {
+                float s;
+                const size_t idx = (zoff(m, s) + ocol) * NV + 1;
+                vcen = Q[idx];
+                if constexpr (EPI == EPI_F32C) { if (D) vcen += D[idx]; }
+            }
+
+            const int sm2 = slot(m - 2), s0 = slot(m - 1), s1 = slot(m), s2 = slot(m + 1), sp2 = slot(m + 2);
+            T fpx[NS], fmx[NS], fpy[NS], fmz[NS], fpz[NS];
+#pragma unroll
+            for (int v = 0; v < NS; ++v) {
+                const T(*P0)[LX] = ring[s0][v];
+                const T(*P1)[LX] = ring[s1][v];
+                const T(*P2)[LX] = ring[s2][v];
+                const T(*M2)[LX] = ring[sm2][v];
+                const T(*Q2)[LX] = ring[sp2][v];
+                // stencil slot q -> LDS (slot order of cweno_weights)
+                auto get = [&](int q) -> T {
+                    switch (q) {
+                    case 0: return P1[ly][lx];
+                    case 1: return P1[ly][lx - 2];
+                    case 2: return P1[ly][lx - 1];
+                    case 3: return P1[ly][lx + 1];
+                    case 4: return P1[ly][lx + 2];
+                    case 5: return P1[ly - 2][lx];
+                    case 6: return P1[ly - 1][lx];
+                    case 7: return P1[ly + 1][lx];
+                    case 8: return P1[ly + 2][lx];
+                    case 9: return M2[ly][lx];
+                    case 10: return P0[ly][lx];
+                    case 11: return P2[ly][lx];
+                    case 12: return Q2[ly][lx];
+                    case 13: return P1[ly + 1][lx + 1];   // xy: ++ +- -+ --
+                    case 14: return P1[ly - 1][lx + 1];
+                    case 15: return P1[ly + 1][lx - 1];
+                    case 16: return P1[ly - 1][lx - 1];
+                    case 17: return P2[ly + 1][lx];       // yz (y,z): ++ +- -+ --
+                    case 18: return P0[ly + 1][lx];
+                    case 19: return P2[ly - 1][lx];
+                    case 20: return P0[ly - 1][lx];
+                    case 21: return P2[ly][lx + 1];       // zx (z,x): ++ +- -+ --
+                    case 22: return P2[ly][lx - 1];
+                    case 23: return P0[ly][lx + 1];
+                    case 24: return P0[ly][lx - 1];
+                    case 25: return P2[ly + 1][lx + 1];   // corners (x,y,z): +++ -++ +-+ ++- --+ -+- +-- ---
+                    case 26: return P2[ly + 1][lx - 1];
+                    case 27: return P2[ly - 1][lx + 1];
+                    case 28: return P0[ly + 1][lx + 1];
+                    case 29: return P2[ly - 1][lx - 1];
+                    case 30: return P0[ly + 1][lx - 1];
+                    case 31: return P0[ly - 1][lx + 1];
+                    default: return P0[ly - 1][lx - 1];
+                    }
+                };
+
+                Axis3<T> ax;
+                bool full = true;
+                if constexpr (SEL) full = (fl >> v) & 1;
+                if (full) {
+                    cweno_axes_lds<T>(get, p, ax);
+                } else {
+                    axis_linear(get(1), get(2), get(0), get(3), get(4), ax.c1[0], ax.c2[0], ax.c3[0]);
+                    axis_linear(get(5), get(6), get(0), get(7), get(8), ax.c1[1], ax.c2[1], ax.c3[1]);
+                    axis_linear(get(9), get(10), get(0), get(11), get(12), ax.c1[2], ax.c2[2], ax.c3[2]);
+                }
+                T m3[3], p3[3];
+                face_q<T, SYS>(get(0), ax, m3, p3);
+                fmx[v] = m3[0]; fpx[v] = p3[0];
+                s_fym[v][ty][tx] = m3[1];
+                fpy[v] = p3[1];
+                fmz[v] = m3[2]; fpz[v] = p3[2];
+                asm volatile("" ::: "memory");   // one component's stencil live at a time
+            }
+            __syncthreads();   // B1: -y states visible; every read of plane m-1 is done
+
+            if (pf) store_plane(m + 3, nxt);   // into the slot of plane m-2
+
+            T Fx[NS], Fy[NS];
+            {
+                T Lf[NS], Rf[NS];
+#pragma unroll
+                for (int v = 0; v < NS; ++v) { Lf[v] = fpx[v]; Rf[v] = __shfl_down(fmx[v], 1, 32); }
+                face_flux<T, SYS, NS>(0, Lf, Rf, K, irho, sL, Fx);
+#pragma unroll
+                for (int v = 0; v < NS; ++v) { Lf[v] = fpy[v]; Rf[v] = ty + 1 < BY ? s_fym[v][ty + 1][tx] : T(0); }
+                face_flux<T, SYS, NS>(1, Lf, Rf, K, irho, sL, Fy);
             }
 #pragma unroll
-            for (int v = 0; v < NS; ++v) Fz_prev[v] = Fz[v];
-        }
+            for (int v = 0; v < NS; ++v) s_Fy[v][ty][tx] = Fy[v];
+            __syncthreads();   // B2: fluxes and plane m+2 visible
+
+            T dxy[NS];
 #pragma unroll
-        for (int v = 0; v < NS; ++v) { zp_prev[v] = fpz[v]; dxy_prev[v] = dxy[v]; }
-        vcen_prev = vcen;
-    }
+            for (int v = 0; v < NS; ++v) {
+                const T Fxm = __shfl_up(Fx[v], 1, 32);
+                const T Fym = ty > 0 ? s_Fy[v][ty - 1][tx] : T(0);
+                dxy[v] = (Fx[v] - Fxm) + (Fy[v] - Fym);
+            }
+
+            if (m >= zb) {
+                T Fz[NS], Lf[NS], Rf[NS];
+#pragma unroll
+                for (int v = 0; v < NS; ++v) { Lf[v] = zp_prev[v]; Rf[v] = fmz[v]; }
+                face_flux<T, SYS, NS>(2, Lf, Rf, K, irho, sL, Fz);
+                if (fin) {
+                    Cell<T, NV> r;
+                    if constexpr (SYS == ACX_ACOUSTIC4) {
+                        const T ih = T(p.inv_h);
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) r.v[v] = -ih * (dxy_prev[v] + (Fz[v] - Fz_prev[v]));
+                    } else {
+                        r.v[0] = vcen_prev;
+                        r.v[1] = T(p.c2) * (dxy_prev[0] + (Fz[0] - Fz_prev[0]));
+                    }
+                    Cell<T, NV> o;
+                    if constexpr (EPI == EPI_RHS) {
+                        o = r;
+                    } else if constexpr (EPI == EPI_AXPY || EPI == EPI_COMB) {
+                        // Q of the own (in-domain) cell of plane m-1 is still in the ring
+                        Cell<T, NV> q;
+                        if constexpr (NS == NV) {
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) q.v[v] = ring[s0][v][ly][lx];
+                        } else {
+                            q = load_cell<T, NV>(Q, cell);
+                        }
+                        if constexpr (EPI == EPI_AXPY) {
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) o.v[v] = q.v[v] + T(p.c) * r.v[v];
+                        } else {
+                            e0 = load_cell<T, NV>(U0, cell);
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) o.v[v] = T(p.a) * e0.v[v] + T(p.b) * q.v[v] + T(p.c) * r.v[v];
+                        }
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) o.v[v] = (has_d ? T(p.b) * e0.v[v] : T(0)) + T(p.c) * r.v[v];
+                        if (p.final_stage) {
+                            Cell<T, NV> nh, nl;
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) {
+                                float t = (float)e1.v[v] + (float)o.v[v];
+                                float sm, er;
+                                two_sum((float)e2.v[v], t, sm, er);
+                                nh.v[v] = sm;
+                                nl.v[v] = er;
+                            }
+                            store_cell<T, NV>(OUT2, cell, nl);
+                            o = nh;
+                        }
+                    }
+                    if (p.nf) finite_acc<T, NV>(nfbad, o);
+                    store_cell<T, NV>(OUT, cell, o);
+                }
+#pragma unroll
+                for (int v = 0; v < NS; ++v) Fz_prev[v] = Fz[v];
+            }
+#pragma unroll
+            for (int v = 0; v < NS; ++v) { zp_prev[v] = fpz[v]; dxy_prev[v] = dxy[v]; }
+            vcen_prev = vcen;
+        }
+        if constexpr (SEL) {
+            __syncthreads();   // the next tile's prologue rewrites the ring
+            item += gridDim.x;
+        }
+    } while (SEL && item < nitems);
     watch_flush(p.nf, nfbad);
 }
 
@@ -445,6 +456,36 @@ __global__ void __launch_bounds__(256) k_tile_mark(const uint8_t *__restrict__ f
     }
 }
 
+// The marked tiles as a list: list[0] = their number, list[1 ..] their
+// indices in ascending order.  One block walks the mask in chunks of 1024
+// entries; a chunk's marked entries are placed by a wavefront ballot and the
+// counts of the wavefronts before it.
+constexpr int TL_NT = 1024;
+
+__global__ void __launch_bounds__(TL_NT) k_tile_list(const uint8_t *__restrict__ mask, int tiles, int *__restrict__ list)
+{
+    __shared__ int wcount[TL_NT / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int base = 0;
+    for (int c0 = 0; c0 < tiles; c0 += TL_NT) {
+        const int t = c0 + (int)threadIdx.x;
+        const bool on = t < tiles && mask[t] != 0;
+        const unsigned long long b = __ballot(on);
+        if (lane == 0) wcount[w] = __popcll(b);
+        __syncthreads();
+        int before = base, total = 0;
+#pragma unroll
+        for (int k = 0; k < TL_NT / 64; ++k) {
+            before += k < w ? wcount[k] : 0;
+            total += wcount[k];
+        }
+        if (on) list[1 + before + __popcll(b & ((1ull << lane) - 1ull))] = t;
+        base += total;
+        __syncthreads();   // wcount is rewritten by the next chunk
+    }
+    if (threadIdx.x == 0) list[0] = base;
+}
+
 size_t cweno_sel_tiles(const CwArgs &a)
 {
     const size_t ntx = (a.nx + CW_BX - 3) / (CW_BX - 2), nty = (a.ny + CW_BY - 3) / (CW_BY - 2);
@@ -455,9 +496,15 @@ int launch_cweno_tilemask(const CwArgs &a, const uint8_t *flags, uint8_t *mask, 
 {
     const int ntx = (a.nx + CW_BX - 3) / (CW_BX - 2), nty = (a.ny + CW_BY - 3) / (CW_BY - 2);
     const int ntz = (a.nz + a.zc - 1) / a.zc;
+    const size_t tiles = cweno_sel_tiles(a);
+    if (tiles > (size_t)INT32_MAX - 1) return set_error(ACX_EUNSUPPORTED, "tile mask: %zu patch tiles", tiles);
     hipLaunchKernelGGL(k_tile_mark, dim3((unsigned)((size_t)ntx * nty * ((ntz + TM_ZT - 1) / TM_ZT))), dim3(256), 0, s,
                        flags, a.nx, a.ny, a.nz, a.zc, ntx, nty, ntz, mask);
-    return hip_check(hipGetLastError(), "tile mask launch");
+    int rc = hip_check(hipGetLastError(), "tile mask launch");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_tile_list, dim3(1), dim3(TL_NT), 0, s, mask, (int)tiles,
+                       reinterpret_cast<int *>(mask + sel_list_offset(tiles)));
+    return hip_check(hipGetLastError(), "tile list launch");
 }
 
 template <typename T, int SYS, int EPI, bool SEL, bool WALLS>
@@ -484,6 +531,20 @@ static int launch_cw_tw(const CwArgs &a, hipStream_t s)
         b.zc = auto_zc(b.ntx, b.nty, a.nz, device_cus() * per_cu, 4);
     }
     b.ntz = (a.nz + b.zc - 1) / b.zc;
+    size_t blocks = (size_t)b.ntx * b.nty * b.ntz;
+    if (SEL && b.tlist) {
+        // the marked tiles of the list, at most one round of resident blocks
+        static int per_cu_sel = 0;
+        if (!per_cu_sel) {
+            int n = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &n, reinterpret_cast<const void *>(&k_cweno<T, SYS, BX, BY, EPI, SEL, WALLS>), BX * BY,
+                    CwLds<T, SYS == ACX_ACOUSTIC4 ? 4 : 1, BX, BY>::bytes) != hipSuccess || n <= 0)
+                n = 1;
+            per_cu_sel = n;
+        }
+        blocks = std::min(blocks, (size_t)device_cus() * per_cu_sel);
+    }
     constexpr int NS = SYS == ACX_ACOUSTIC4 ? 4 : 1;
     constexpr size_t lds = CwLds<T, NS, BX, BY>::bytes;
     static bool attr_set = false;
@@ -495,7 +556,7 @@ static int launch_cw_tw(const CwArgs &a, hipStream_t s)
         attr_set = true;
     }
     dim3 blk(BX, BY, 1);
-    dim3 grd(b.ntx * b.nty * b.ntz, 1, 1);
+    dim3 grd((unsigned)blocks, 1, 1);
     hipLaunchKernelGGL((k_cweno<T, SYS, BX, BY, EPI, SEL, WALLS>), grd, blk, lds, s, b);
     return hip_check(hipGetLastError(), "k_cweno launch");
 }

@@ -34,7 +34,7 @@ struct CwArgs {
     void *out;
     void *out2;
     const uint8_t *flags;   // selective mode: per-cell bitmask, bit v set => full CWENO for component v
-    const uint8_t *tilemask;   // selective mode: tiles with a zero entry are left to the linear pass
+    const int *tlist;          // selective mode: marked tiles (count, then indices); null: every tile
     int nx, ny, nz, zc;
     int ntx, nty, ntz;
     int bc[3];
@@ -50,9 +50,13 @@ int launch_linear(const LinArgs &a, int sys, int epi, bool fp32, hipStream_t s);
 // a periodic one-plane grid (acx_step2d.hip); IN and OUT distinct.
 int launch_step2d(const LinArgs &a, const double *IN, double *OUT, double dt, hipStream_t s);
 int launch_cweno(const CwArgs &a, int sys, int epi, bool fp32, bool selective, hipStream_t s);
-// Tiles of the FP32 selective CWENO launch for a.zc: count, and a mask that is
-// 1 for every tile holding a flagged cell or a face neighbour of one.
+// Tiles of the FP32 selective CWENO launch for a.zc: count, and the patch
+// buffer for the flags: one byte per tile (1 for every tile holding a flagged
+// cell or a face neighbour of one), then, from the next 4-byte boundary, the
+// marked tiles as int32 (their number, then their indices in ascending order).
 size_t cweno_sel_tiles(const CwArgs &a);
+constexpr size_t sel_list_offset(size_t tiles) { return (tiles + 3) & ~(size_t)3; }
+constexpr size_t sel_buffer_bytes(size_t tiles) { return sel_list_offset(tiles) + 4 * (tiles + 1); }
 int launch_cweno_tilemask(const CwArgs &a, const uint8_t *flags, uint8_t *mask, hipStream_t s);
 
 int set_error(int code, const char *fmt, ...);

@@ -108,20 +108,20 @@ int acx_flags_count(const uint8_t *flags, size_t ncells, long long *bits, void *
 /* One compensated FP32 SSP-RK3 step with the selective reconstruction:
  * full CWENO4 (grid gammas) where the flag bit of a component is set, the
  * linear 4th-order polynomial elsewhere.  Each stage runs the linear-scheme
- * kernel over every cell, then the CWENO kernel over the tiles (30 x 14 x 4
- * cells; ACX_ZC_PATCH sets the 4, read once per process) that hold a flagged
- * cell or a face neighbour of one.  The tile mask is made from the flags on
- * every call, into a buffer taken from the stream-ordered pool of `stream`
- * for the call.  ACOUSTIC4. */
+ * kernel over every cell, then the CWENO kernel over the tiles (30 x 14 x 8
+ * cells; ACX_ZC_PATCH sets the 8, read once per process) that hold a flagged
+ * cell or a face neighbour of one, visiting only those tiles.  The tile mask
+ * (acx_sel_mask) is made from the flags on every call, into a buffer taken
+ * from the stream-ordered pool of `stream` for the call.  ACOUSTIC4. */
 int acx_ssp_rks3_step_f32c_sel(const acx_grid *g, const float *HI, const float *LO, float *HI_OUT,
                                float *LO_OUT, float *D0, float *D1, const uint8_t *flags, double dt, void *stream);
 
 /* acx_ssp_rks3_step_f32c_sel without any allocation.  No counterpart in the
  * reference; for callers that cannot allocate device memory (code under HIP
  * graph capture, serving loops, harnesses that forbid hipMalloc).  The tile
- * mask is kept in a fixed static device array of ACX_SEL_NOALLOC_MAX_TILES
- * bytes per GPU (held until the process ends): grids with more patch tiles
- * (acx_sel_mask_bytes) are rejected with ACX_EINVAL.  Calls on one GPU share
+ * mask is kept in a fixed static device array sized for
+ * ACX_SEL_NOALLOC_MAX_TILES patch tiles (1.25 MiB per GPU, held until the
+ * process ends): grids with more patch tiles are rejected with ACX_EINVAL.  Calls on one GPU share
  * that array, so calls that overlap on one GPU (other streams or threads)
  * race; the mask is rewritten on every call.  The non-finite watch applies
  * once another entry point has created the device's watch word.  Results
@@ -131,9 +131,12 @@ int acx_ssp_rks3_step_f32c_sel_noalloc(const acx_grid *g, const float *HI, const
                                        float *LO_OUT, float *D0, float *D1, const uint8_t *flags, double dt,
                                        void *stream);
 
-/* The tile mask of the selective step for these flags: acx_sel_mask_bytes(g)
- * bytes (0 for an invalid grid), one per patch tile, 1 where the tile holds a
- * flagged cell or a face neighbour of one. */
+/* The tile mask of the selective step for these flags: a buffer of
+ * acx_sel_mask_bytes(g) bytes (0 for an invalid grid), 4-byte aligned.  It
+ * holds one byte per patch tile, 1 where the tile holds a flagged cell or a
+ * face neighbour of one, then, from the next 4-byte boundary, the marked
+ * tiles as int32: their number, then their indices in ascending order (the
+ * list the CWENO pass walks).  Masks are made only by acx_sel_mask. */
 size_t acx_sel_mask_bytes(const acx_grid *g);
 int acx_sel_mask(const acx_grid *g, const uint8_t *flags, uint8_t *mask, void *stream);
 

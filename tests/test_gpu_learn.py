@@ -296,8 +296,10 @@ def test_tile_patch_equals_full_selective_kernel(acx, torch_gpu, monkeypatch):
 
 def test_tile_mask_and_masked_step(acx, torch_gpu):
     """acx_sel_mask marks a patch tile (30 x 14 x zc) exactly when it holds a
-    flagged cell or a periodic face neighbour of one; the step with that mask
-    is bitwise the step that makes the mask itself."""
+    flagged cell or a periodic face neighbour of one, and lists the marked
+    tiles in ascending order after the mask; the step with that mask is
+    bitwise the step that makes the mask itself."""
+    import os
     torch = torch_gpu
     from acoustic_petsc_amd import learn
     g = acx.Grid(n=(47, 33, 21), h=1.0 / 47, gamma_hi=0.85, gamma_lo=0.85)
@@ -306,12 +308,16 @@ def test_tile_mask_and_masked_step(acx, torch_gpu):
     idx = rng.choice(g.ncells, 40, replace=False)
     fl[idx] = rng.integers(1, 16, idx.size)
     flags = torch.from_numpy(fl).cuda()
-    mask = acx.sel_mask(g, flags).cpu().numpy()
+    buf = acx.sel_mask(g, flags).cpu().numpy()
     nx, ny, nz = g.n
     ntx, nty = -(-nx // 30), -(-ny // 14)
-    assert mask.size % (ntx * nty) == 0
-    ntz = mask.size // (ntx * nty)
-    zc = next(c for c in range(1, nz + 1) if -(-nz // c) == ntz)   # the tiling's z-chunk
+    zc = min(max(int(os.environ.get("ACX_ZC_PATCH", "8")), 1), nz)   # the tiling's z-chunk
+    ntiles = ntx * nty * -(-nz // zc)
+    off = (ntiles + 3) // 4 * 4
+    assert buf.size == off + 4 * (ntiles + 1)
+    mask = buf[:ntiles]
+    tl = buf[off:].view(np.int32)
+    assert tl[0] == int(mask.sum()) and np.array_equal(tl[1:1 + tl[0]], np.nonzero(mask)[0])
     want = np.zeros(mask.size, np.uint8)
     f3 = fl.reshape(nz, ny, nx)
     for k, j, i in zip(*np.nonzero(f3)):
@@ -323,10 +329,39 @@ def test_tile_mask_and_masked_step(acx, torch_gpu):
     U = torch.from_numpy(learn.steep_pulse_field(g, 5)).cuda()
     dt = acx.cfl_dt(g, 0.4)
     a, b = acx.CompensatedState(g, U), acx.CompensatedState(g, U)
-    mt = torch.from_numpy(mask).cuda()
+    mt = torch.from_numpy(buf).cuda()
     for _ in range(2):
         a.step(dt, flags=flags)
         b.step(dt, flags=flags, mask=mt)
+    assert torch.equal(a.hi, b.hi) and torch.equal(a.lo, b.lo)
+
+
+def test_patch_list_longer_than_the_grid_of_blocks(acx, torch_gpu, monkeypatch):
+    """Every cell flagged on a grid of 9 x 10 x 8 = 720 patch tiles, more than
+    the resident blocks of the patch launch (2 per CU), so blocks take several
+    list entries each: every cell's output then comes from the patch, and the
+    step is bitwise the CWENO kernel run over every tile."""
+    import os
+    torch = torch_gpu
+    from acoustic_petsc_amd import learn
+    g = acx.Grid(n=(256, 140, 64), h=1.0 / 256, gamma_hi=0.85, gamma_lo=0.85)
+    U = torch.from_numpy(learn.steep_pulse_field(g, 23)).cuda()
+    rng = np.random.default_rng(6)
+    flags = torch.from_numpy(rng.integers(1, 16, g.ncells).astype(np.uint8)).cuda()
+    mask = acx.sel_mask(g, flags)
+    ntiles = 9 * 10 * -(-64 // min(max(int(os.environ.get("ACX_ZC_PATCH", "8")), 1), 64))
+    off = (ntiles + 3) // 4 * 4
+    count = int(mask[off:off + 4].view(torch.int32).item())
+    print(f"[case] all cells flagged: {count} of {ntiles} patch tiles listed", flush=True)
+    assert count == ntiles and ntiles > 2 * torch.cuda.get_device_properties(0).multi_processor_count
+    dt = acx.cfl_dt(g, 0.4)
+    a, b = acx.CompensatedState(g, U), acx.CompensatedState(g, U)
+    for _ in range(2):
+        a.step(dt, flags=flags, mask=mask)
+    monkeypatch.setenv("ACX_SEL_FULL", "1")
+    for _ in range(2):
+        b.step(dt, flags=flags)
+    monkeypatch.delenv("ACX_SEL_FULL")
     assert torch.equal(a.hi, b.hi) and torch.equal(a.lo, b.lo)
 
 

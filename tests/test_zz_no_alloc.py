@@ -76,10 +76,10 @@ def test_no_alloc_selective_step_equals_the_allocating_one(acx, torch_gpu):
 def test_no_alloc_selective_step_rejects_grids_beyond_its_scratch(acx, torch_gpu):
     torch = torch_gpu
     L = acx.lib()
-    # 30 x 14 x 4-cell patch tiles: 1024^2 x 1024 cells need 35 x 74 x 256 = 663040 > 262144 tiles
+    # 30 x 14 x 8-cell patch tiles: 1024^2 x 1024 cells need 35 x 74 x 128 = 331520 > 262144 tiles
     grid = acx.Grid(n=(1024, 1024, 1024), h=1.0 / 1024, system=acx.ACOUSTIC4, gamma_hi=0.85, gamma_lo=0.85)
     g = grid.c_struct()
-    assert L.acx_sel_mask_bytes(C.byref(g)) > 262144
+    assert L.acx_sel_mask_bytes(C.byref(g)) > 5 * 262144 + 4   # a byte and an int32 per tile, and the count
     fn = L.acx_ssp_rks3_step_f32c_sel_noalloc
     fn.argtypes = [C.c_void_p] + [C.c_void_p] * 7 + [C.c_double, C.c_void_p]
     fn.restype = C.c_int
