@@ -83,13 +83,16 @@ struct CwLds {
     static constexpr size_t bytes = (ring + 2ull * NS * BY * BX) * sizeof(T);
 };
 
-// Waves per SIMD the register allocation aims for: the selective patch runs a
-// few hundred marked tiles, so two blocks per CU (4 waves per SIMD, at most
-// 128 VGPRs) let one launch hold them all at once; the full-grid kernel keeps
-// the registers (one block per CU).
+// Waves per SIMD the register allocation aims for.  The selective patch
+// marches a few hundred marked tiles of zc + 2 planes each, and the time of one
+// tile's march is what counts: with 2 waves per SIMD (one block per CU, no
+// spills) against 4 (two blocks per CU, at most 128 VGPRs, 60 spilled), a
+// steep pulse's selective step took 1.146 -> 1.076 ms at 256^3 (232 marked
+// tiles, one round either way) and the same 3.14 ms at 384^3 (508 tiles: two
+// rounds against one).  The full-grid kernel keeps the registers too.
 template <bool SEL>
 struct CwWaves {
-    static constexpr int min_per_simd = SEL ? 4 : 1;
+    static constexpr int min_per_simd = SEL ? 2 : 1;
 };
 
 // WALLS = false: every axis periodic (wrap-only index maps, no ghost signs).
@@ -213,16 +216,24 @@ k_cweno(CwArgs p)
         const int lx = tx + 2, ly = ty + 2;
 
         uint8_t fl_next = 0xff;
+        // Selective patch: plane m+3 is requested in iteration m-1, right
+        // after the plane it replaces in registers went to LDS, so its load
+        // has a whole iteration to land (with the linear polynomial at most
+        // of a patch tile's cells the iteration is too short to hide a load
+        // issued at its start).
+        Raw nxt_sel[PER];
         if constexpr (SEL) {
             float s;
             fl_next = p.flags[zoff(zb - 1, s) + ocol];
+            load_plane(zb + 2, nxt_sel);
         }
         for (int m = zb - 1; m <= ze; ++m) {
             // prefetch plane m+3 of the region, the flags of plane m+1 and the
             // epilogue operands of plane m-1
-            Raw nxt[PER];
+            Raw nxt_full[PER];
+            Raw(&nxt)[PER] = SEL ? nxt_sel : nxt_full;
             const bool pf = m + 3 <= ze + 2;
-            if (pf) load_plane(m + 3, nxt);
+            if (!SEL && pf) load_plane(m + 3, nxt);
             const uint8_t fl = fl_next;
             if constexpr (SEL) {
                 float s;
@@ -319,6 +330,7 @@ k_cweno(CwArgs p)
             __syncthreads();   // B1: -y states visible; every read of plane m-1 is done
 
             if (pf) store_plane(m + 3, nxt);   // into the slot of plane m-2
+            if (SEL && m + 4 <= ze + 2) load_plane(m + 4, nxt);
 
             T Fx[NS], Fy[NS];
             {
