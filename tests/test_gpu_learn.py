@@ -336,6 +336,30 @@ def test_tile_mask_and_masked_step(acx, torch_gpu):
     assert torch.equal(a.hi, b.hi) and torch.equal(a.lo, b.lo)
 
 
+def test_tile_list_over_several_chunks(acx, torch_gpu):
+    """The list of marked tiles over a mask of 10 x 20 x 8 = 1600 tiles (the
+    listing block walks it in chunks of 1024): the count and the ascending
+    indices of the marked entries, for a sparse and a dense marking."""
+    import os
+    torch = torch_gpu
+    g = acx.Grid(n=(300, 280, 64), h=1.0 / 300, gamma_hi=0.85, gamma_lo=0.85)
+    zc = min(max(int(os.environ.get("ACX_ZC_PATCH", "8")), 1), 64)
+    ntiles = 10 * 20 * -(-64 // zc)
+    off = (ntiles + 3) // 4 * 4
+    rng = np.random.default_rng(77)
+    for frac in (2e-6, 3e-4):
+        fl = np.zeros(g.ncells, np.uint8)
+        idx = rng.choice(g.ncells, max(1, int(frac * g.ncells)), replace=False)
+        fl[idx] = 1
+        buf = acx.sel_mask(g, torch.from_numpy(fl).cuda()).cpu().numpy()
+        assert buf.size == off + 4 * (ntiles + 1)
+        mask = buf[:ntiles]
+        tl = buf[off:].view(np.int32)
+        print(f"[case] tile list: {ntiles} tiles, {int(mask.sum())} marked", flush=True)
+        assert tl[0] == int(mask.sum())
+        assert np.array_equal(tl[1:1 + tl[0]], np.nonzero(mask)[0])
+
+
 def test_patch_list_longer_than_the_grid_of_blocks(acx, torch_gpu, monkeypatch):
     """Every cell flagged on a grid of 9 x 10 x 8 = 720 patch tiles, more than
     the resident blocks of the patch launch (2 per CU), so blocks take several
