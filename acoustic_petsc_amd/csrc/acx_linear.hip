@@ -139,7 +139,7 @@ static int launch_linear_v(const LinArgs &a, hipStream_t s)
         // smallest z-chunk that fills one round, and with four co-resident
         // blocks of the WAVE2 halo-ring kernel that was 32 planes, whose
         // z-halo re-reads cost more than the round saved (measured at 256^3:
-        // WAVE2 step 0.461 ms with the COMB stage at 32 planes, 0.452 at 64).
+        // WAVE2 step 0.458 ms with the COMB stage at 32 planes, 0.450 at 64).
         b.zc = auto_zc(b.ntx, b.nty, a.nz, device_cus() * (per_cu < 2 ? per_cu : 2), 10);
     }
     b.ntz = (a.nz + b.zc - 1) / b.zc;
