@@ -124,19 +124,6 @@ __device__ __forceinline__ void finite_acc(bool &bad, const Cell<T, NV> &c)
     for (int v = 0; v < NV; ++v) bad |= !__builtin_isfinite(c.v[v]);
 }
 
-// Workgroup barrier for data exchanged through LDS only: each wavefront's LDS
-// accesses are complete (lgkmcnt 0) when it arrives, while its global loads
-// and stores stay in flight (__syncthreads also waits for those, vmcnt 0,
-// which drains every load requested ahead of the barrier).  The empty asm
-// statements keep the compiler from moving memory accesses across it.
-__device__ __forceinline__ void lds_barrier()
-{
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_waitcnt(0xC07F);   // gfx9 encoding: lgkmcnt 0; vmcnt 63, expcnt 7 (no wait)
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-}
-
 __device__ __forceinline__ void watch_flush(unsigned *nf, bool bad)
 {
     if (!nf) return;

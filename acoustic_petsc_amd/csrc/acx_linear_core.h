@@ -207,7 +207,7 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
 #pragma unroll
                     for (int v = 0; v < NS; ++v) lds[buf][v][hy][hx] = hcur.v[v];
                 }
-                lds_barrier();   // the loads requested ahead stay in flight
+                __syncthreads();
                 if constexpr (SYS == ACX_ACOUSTIC4) {
                     T accB[4], dAx = T(0), dAy = T(0), dAvel = T(0);
 #pragma unroll
