@@ -217,10 +217,11 @@ k_cweno(CwArgs p)
 
         uint8_t fl_next = 0xff;
         // Selective patch: plane m+3 is requested in iteration m-1, right
-        // after the plane it replaces in registers went to LDS, so its load
-        // has a whole iteration to land (with the linear polynomial at most
-        // of a patch tile's cells the iteration is too short to hide a load
-        // issued at its start).
+        // after the plane before it went to LDS, instead of at the start of
+        // iteration m.  (The compiler waits for these loads together with the
+        // epilogue operands, before the results of iteration m-1 are stored,
+        // so they overlap that iteration's fluxes.)  Measured with one block
+        // per CU on a steep pulse at 256^3: selective step 1.135 -> 1.076 ms.
         Raw nxt_sel[PER];
         if constexpr (SEL) {
             float s;
