@@ -267,13 +267,17 @@ def test_trained_indicator_gain(acx, torch_gpu):
     assert e_sel < 1e-3 * e_lin
 
 
-def test_tile_patch_equals_full_selective_kernel(acx, torch_gpu, monkeypatch):
+@pytest.mark.parametrize("walls", [False, True])
+def test_tile_patch_equals_full_selective_kernel(acx, torch_gpu, monkeypatch, walls):
     """The selective step (linear pass + CWENO on flagged tiles) equals the
-    CWENO kernel run over every tile with the same flags, to FP32 rounding."""
+    CWENO kernel run over every tile with the same flags, to FP32 rounding;
+    also with walls (a reflective x axis and a transmissive z axis: the
+    kernels' wall instantiations)."""
     torch = torch_gpu
     from acoustic_petsc_amd import learn
     N = 40
-    g = acx.Grid(n=(N, N + 4, N - 4), h=1.0 / N, gamma_hi=0.85, gamma_lo=0.85)
+    bc = (acx.REFLECTIVE, acx.PERIODIC, acx.TRANSMISSIVE) if walls else (acx.PERIODIC,) * 3
+    g = acx.Grid(n=(N, N + 4, N - 4), h=1.0 / N, gamma_hi=0.85, gamma_lo=0.85, bc=bc)
     U = torch.from_numpy(learn.steep_pulse_field(g, 21)).cuda()
     rng = np.random.default_rng(4)
     fl = np.zeros(g.ncells, np.uint8)
@@ -290,7 +294,7 @@ def test_tile_patch_equals_full_selective_kernel(acx, torch_gpu, monkeypatch):
     monkeypatch.delenv("ACX_SEL_FULL")
     ra, rb = a.value64().cpu().numpy(), b.value64().cpu().numpy()
     err = np.linalg.norm(ra - rb) / np.linalg.norm(rb)
-    print(f"[case] tile patch vs full selective kernel: {err:.2e}", flush=True)
+    print(f"[case] tile patch vs full selective kernel (walls {walls}): {err:.2e}", flush=True)
     assert err < 1e-6
 
 
