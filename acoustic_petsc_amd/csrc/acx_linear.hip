@@ -185,11 +185,9 @@ int launch_linear(const LinArgs &a, int sys, int epi, bool fp32, hipStream_t s)
     if (fp32) {
         // the stage that ends the step has its own instantiation (EPI_F32CF)
         if (sys == ACX_ACOUSTIC4) {
-            if (epi == EPI_F32C) {
-                if (a.final_stage) return launch_linear_t<float, ACX_ACOUSTIC4, EPI_F32CF>(a, s);
-                return a.d ? launch_linear_t<float, ACX_ACOUSTIC4, EPI_F32C>(a, s)
-                           : launch_linear_t<float, ACX_ACOUSTIC4, EPI_F32C0>(a, s);
-            }
+            if (epi == EPI_F32C)
+                return a.final_stage ? launch_linear_t<float, ACX_ACOUSTIC4, EPI_F32CF>(a, s)
+                                     : launch_linear_t<float, ACX_ACOUSTIC4, EPI_F32C>(a, s);
         } else {
             if (epi == EPI_F32C)
                 return a.final_stage ? launch_linear_t<float, ACX_WAVE2, EPI_F32CF>(a, s)

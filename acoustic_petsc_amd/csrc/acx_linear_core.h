@@ -20,10 +20,7 @@ struct LinTile {
 // HI and LO; p.final_stage is implied), instantiated on its own so that the
 // lead-plane march can keep HI of the last four planes in registers for the
 // epilogue's exact sum instead of loading it again three planes later.
-// EPI_F32C0: the compensated FP32 stage that starts the step (no increment D
-// to read: stencils on HI alone), instantiated on its own so that the queue
-// of D and the run-time tests of its presence drop out of its march.
-enum { EPI_RHS = 0, EPI_AXPY = 1, EPI_COMB = 2, EPI_F32C = 3, EPI_F32CF = 4, EPI_F32C0 = 5 };
+enum { EPI_RHS = 0, EPI_AXPY = 1, EPI_COMB = 2, EPI_F32C = 3, EPI_F32CF = 4 };
 
 // One tile: columns [x0, x0+BX) x [y0, y0+BY), planes [zb, ze) of the grid
 // described by p (shared by the per-grid kernel and the ensemble kernel).
@@ -98,9 +95,8 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
     const T inv48h = T(p.inv48h), Kc = T(p.K), irho = T(p.invrho), sLLF = T(p.s);
     const T lap = T(p.lapc);   // WAVE2: c^2 / (48 h^2)
     const int jbeg = zb - 3, jend = ze + 2;
-    constexpr bool F32C = EPI == EPI_F32C || EPI == EPI_F32CF || EPI == EPI_F32C0;
-    constexpr bool DQ = F32C && EPI != EPI_F32C0;   // an increment D is read and queued
-    const bool has_d = DQ && D != nullptr;
+    constexpr bool F32C = EPI == EPI_F32C || EPI == EPI_F32CF;
+    const bool has_d = F32C && D != nullptr;
     const bool fin_stage = EPI == EPI_F32CF || (EPI == EPI_F32C && p.final_stage);
 
     struct Slot {
@@ -138,11 +134,11 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
     T xyq[4][NP];        // in-plane RHS parts of planes j-3 .. j
     // F32C: the increment D of planes j-3 .. j, kept from the stencil loads for
     // the epilogue of plane j-3 (a second load of it three planes later missed L2)
-    Cell<T, NV> dq[DQ ? 4 : 1];
+    Cell<T, NV> dq[F32C ? 4 : 1];
     // EPI_F32CF: HI of planes j-3 .. j as loaded (before adding D)
     Cell<T, NV> hq[EPI == EPI_F32CF ? 4 : 1];
 #pragma unroll
-    for (int k = 0; k < (DQ ? 4 : 1); ++k)
+    for (int k = 0; k < (F32C ? 4 : 1); ++k)
 #pragma unroll
         for (int v = 0; v < NV; ++v) dq[k].v[v] = T(0);
 #pragma unroll
@@ -183,7 +179,7 @@ __device__ __forceinline__ void linear_tile(const LinArgs &p, int x0, int y0, in
             if (has_d) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) cur.v[v] += S.d.v[v];
-                if constexpr (DQ) {
+                if constexpr (F32C) {
 #pragma unroll
                     for (int k = 0; k < 3; ++k) dq[k] = dq[k + 1];
                     dq[3] = S.d;
