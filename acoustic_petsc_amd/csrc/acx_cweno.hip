@@ -123,9 +123,10 @@ k_cweno(CwArgs p)
     // ascending order), or every tile without a list; block b takes entries
     // b, b + grid, ...  The marked tiles cluster in space (along a front), and
     // consecutive entries go to consecutive blocks, i.e. round-robin over the
-    // XCDs.  Only marked tiles are visited: a grid of one block per tile that
-    // returned at once on an unmarked one took longer for the empty blocks
-    // than for the marked.  Otherwise one tile per block, XCD-contiguous.
+    // XCDs.  Only marked tiles are visited: with one block per tile of the
+    // grid, the blocks that returned at once on unmarked tiles cost about
+    // 16 us per stage at 256^3 and 21 us at 384^3 (4-plane tiles, measured).
+    // Otherwise one tile per block, XCD-contiguous.
     int item = blockIdx.x;
     const int nitems = SEL ? (p.tlist ? p.tlist[0] : p.ntx * p.nty * p.ntz) : 1;
     if (SEL && item >= nitems) return;
