@@ -222,7 +222,8 @@ k_cweno(CwArgs p)
         // iteration m.  (The compiler waits for these loads together with the
         // epilogue operands, before the results of iteration m-1 are stored,
         // so they overlap that iteration's fluxes.)  Measured with one block
-        // per CU on a steep pulse at 256^3: selective step 1.135 -> 1.076 ms.
+        // per CU on a steep pulse at 256^3, in two runs beside a reference
+        // build that took 1.15-1.16 ms in both: 1.135 -> 1.076 ms per step.
         Raw nxt_sel[PER];
         if constexpr (SEL) {
             float s;
