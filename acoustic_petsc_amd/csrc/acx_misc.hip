@@ -43,11 +43,14 @@ __device__ __forceinline__ void block_reduce(double &s, double &m)
 }
 
 // ---- vector copy ------------------------------------------------------------
-// Y = X with nontemporal 16-byte loads and stores: each thread issues its
-// CP_UNROLL loads before the first store, so a wave keeps 4 KiB in flight;
-// a block covers CP_THREADS * CP_UNROLL pairs.  k_copy1 is the scalar form
-// for pointers off 16-byte alignment and for the odd last element.
-constexpr int CP_THREADS = 256, CP_UNROLL = 4;
+// Y = X with nontemporal 16-byte loads and stores, one pair per thread: a
+// block covers CP_THREADS * CP_UNROLL pairs.  Measured on MI355X (read +
+// write rate, three alternations on one box): a 256^3 FP64 ACOUSTIC4 state
+// 6.29 -> 6.52 TB/s and a 384^3 one 6.10 -> 6.43 TB/s against four pairs per
+// thread (each thread's loads issued before its first store).  k_copy1 is
+// the scalar form for pointers off 16-byte alignment and for the odd last
+// element.
+constexpr int CP_THREADS = 256, CP_UNROLL = 1;
 
 __global__ void __launch_bounds__(CP_THREADS) k_copy2(const acx_dbl2 *__restrict__ X, acx_dbl2 *__restrict__ Y, size_t n2)
 {
